@@ -1,0 +1,238 @@
+/*
+ * sph_mi355x.h -- C ABI of libsph_mi355x.so, the MI355X (gfx950) particle
+ * step of PySPH's WCSPH hot path.
+ *
+ * Conventions
+ *   - Every pointer inside a struct or argument named *_dev, and every field
+ *     pointer of sph_view, is a DEVICE pointer (e.g. torch.Tensor.data_ptr()).
+ *     Arguments named *_host are host pointers.
+ *   - Every call is asynchronous on `stream` (a hipStream_t passed as void*;
+ *     NULL = the default stream) unless its comment says it synchronises.
+ *   - Return codes: 0 ok; >0 a hipError_t; SPH_EINVAL (-1) bad argument;
+ *     SPH_ECAPACITY (-2) output too small (`needed` is set);
+ *     SPH_EJUMP (-3) a particle moved more than one cell since the previous
+ *     binning; SPH_EUNSUPPORTED (-4) unsupported dimension or dtype.
+ *     sph_last_error() returns the message of the last failure (per thread).
+ *   - The library never frees or keeps caller buffers past a call.
+ *   - Outputs are written, not accumulated: one call over all sources equals
+ *     the reference's zero-then-accumulate (sph_calc.pyx:223-276).  Particles
+ *     whose tag is not LocalReal (0) get 0 (sph_func.pyx:211-234).
+ *
+ * Reference paths are relative to source/pysph/ of pankajp/pysph.
+ */
+#ifndef SPH_MI355X_H
+#define SPH_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SPH_ABI_VERSION 1
+
+#define SPH_OK 0
+#define SPH_EINVAL (-1)
+#define SPH_ECAPACITY (-2)
+#define SPH_EJUMP (-3)
+#define SPH_EUNSUPPORTED (-4)
+
+#define SPH_F32 0
+#define SPH_F64 1
+
+#define SPH_MAX_SRC 4
+
+/* Source roles of the fused momentum pass. */
+#define SPH_SRC_MOMENTUM 1u
+#define SPH_SRC_XSPH 2u
+
+/* Pair arithmetic of the neighbour passes when the storage is FP32:
+ * SPH_PAIR_NATIVE computes in the storage type, SPH_PAIR_F64 evaluates each
+ * pair and accumulates in FP64 (storage stays FP32). */
+#define SPH_PAIR_NATIVE 0
+#define SPH_PAIR_F64 1
+
+/* One ParticleArray on the device (particle_array.pyx:96-1279): SoA, one
+ * contiguous array per property, all real fields of type `dtype`.  Fields a
+ * call does not read may be NULL. */
+typedef struct sph_view {
+    int64_t n;
+    int32_t dtype; /* SPH_F32 or SPH_F64 */
+    int32_t reserved;
+    const void *x, *y, *z, *u, *v, *w, *h, *m, *rho, *p, *cs;
+    const int64_t *tag;        /* NULL: every particle LocalReal */
+    const int32_t *cell;       /* per particle: linear cell from sph_bin */
+    const int32_t *cell_start; /* ncells + 1 offsets from sph_bin */
+} sph_view;
+
+/* The cell grid shared by all arrays of one Particles set.  Cell (i,j,k) of
+ * the reference (floor(x/cell_size), cell.pyx:42-82) is grid cell
+ * (i-lo[0], j-lo[1], k-lo[2]), clamped into [0, dims).  Clamping keeps the
+ * stencil search complete because it never increases a cell distance. */
+typedef struct sph_grid {
+    double cell_size;
+    int32_t lo[3];
+    int32_t dims[3]; /* 1 for unused axes */
+} sph_grid;
+
+/* Parameters of the WCSPH operations.
+ *   TaitEquation(co, ro, gamma)             eos_funcs.pyx:77-92
+ *   MomentumEquation(alpha, beta, eta)      pressure_funcs.pyx:117-134
+ *   XSPHCorrection(eps)                     xsph_funcs.pyx:13-18
+ *   GravityForce(gx, gy, gz)                external_force.pyx:19-34 */
+typedef struct sph_params {
+    int32_t dim;        /* 1, 2 or 3 (CubicSplineKernel(dim)) */
+    int32_t pair_prec;  /* SPH_PAIR_NATIVE or SPH_PAIR_F64 */
+    double radius_scale; /* kernel.radius() == 2 */
+    double co, ro, gamma;
+    double alpha, beta, eta;
+    double eps;
+    double g[3];
+    int32_t has_gravity;
+    int32_t has_xsph;
+} sph_params;
+
+int32_t sph_abi_version(void);
+const char *sph_last_error(void);
+/* Number of visible devices (synchronous; 0 when none). */
+int32_t sph_device_count(void);
+
+/* ---- binning: CellManager.update / Cell.update (cell.pyx:319-432,
+ *      :927-1015) and the cell size rule (cell.pyx:893-925) ---------------- */
+
+/* min/max of x,y,z and max of h over the array -> out_dev[0..6] (double):
+ * xmin,ymin,zmin,xmax,ymax,zmax,hmax.  out_dev must hold SPH_BOUNDS_WORDS
+ * doubles; the words after the first 8 are the reduction's scratch.  Reads
+ * only x,y,z,h. */
+#define SPH_BOUNDS_WORDS 456
+int32_t sph_bounds(void *stream, const sph_view *a, double *out_dev);
+
+/* Workspace bytes sph_bin needs for n particles and ncells cells. */
+size_t sph_bin_workspace_bytes(int64_t n, int64_t ncells);
+
+/* Counting-sort binning by cell key.  Writes, for the NEW (cell-sorted)
+ * layout: perm_dev[p] = the current slot of the particle that goes to slot
+ * p (stable: current-slot order inside a cell), cell_dev[p] its linear
+ * cell, cell_start_dev[ncells+1].  When prev_cell_dev is not NULL it holds
+ * each particle's previous cell in the CURRENT layout; a particle whose cell
+ * moved by more than one in any axis is counted into *jump_count_dev (the
+ * reference's jump tolerance, cell.pyx:1422-1437).  Reads x,y,z. */
+int32_t sph_bin(void *stream, const sph_view *a, const sph_grid *g,
+                int32_t *perm_dev, int32_t *cell_dev, int32_t *cell_start_dev,
+                const int32_t *prev_cell_dev, int32_t *jump_count_dev,
+                void *workspace_dev, size_t workspace_bytes);
+
+/* Gather nfields arrays through perm: dst[f][p] = src[f][perm[p]].
+ * elem_bytes[f] is 4 or 8.  Host arrays of device pointers. */
+int32_t sph_permute(void *stream, int64_t n, const int32_t *perm_dev,
+                    int32_t nfields, const void *const *src_dev_host,
+                    void *const *dst_dev_host, const int32_t *elem_bytes_host);
+
+/* ---- FixedDestNbrParticleLocator.get_nearest_particles (nnps.pyx:458-620)
+ *      as a CSR export: the exact FP64 test
+ *      ((xs-xd)^2 + (ys-yd)^2) + (zs-zd)^2 < (h_d*radius_scale)^2
+ *      (point.pxd:35-37, nnps.pyx:311-354, :552), rows in the destination's
+ *      current layout, entries = src_ids[j] sorted ascending. ------------- */
+
+/* counts_dev[i] = number of neighbours of destination i. */
+int32_t sph_neighbor_counts(void *stream, const sph_grid *g, const sph_view *dst,
+                            const sph_view *src, double radius_scale,
+                            int64_t *counts_dev);
+/* Exclusive scan of counts into offsets_dev[n+1]; synchronises and returns
+ * the total through *total_host. */
+int32_t sph_scan_counts(void *stream, int64_t n, const int64_t *counts_dev,
+                        int64_t *offsets_dev, int64_t *total_host);
+/* Fill the CSR rows.  cap < offsets[n] -> SPH_ECAPACITY with *needed_host. */
+int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
+                          const sph_view *src, double radius_scale,
+                          const int64_t *src_ids_dev, const int64_t *offsets_dev,
+                          int64_t *nbrs_dev, int64_t cap, int64_t total,
+                          int64_t *needed_host);
+
+/* ---- neighbour passes -------------------------------------------------- */
+
+/* SPHRho over all sources (density_funcs.pyx:38-84), optionally followed by
+ * TaitEquation on the destination (eos_funcs.pyx:105-116) when p_dev and
+ * cs_dev are both non-NULL.  `disp_dev` (nullable, one float) is the largest
+ * coordinate displacement of any particle since the arrays were binned; the
+ * stencil ring grows to cover it so that a binning can be reused across an
+ * RK2 stage.  rmax = the largest search radius (radius_scale * max h). */
+int32_t sph_density(void *stream, const sph_grid *g, const sph_params *prm,
+                    const sph_view *dst, int32_t nsrc, const sph_view *srcs,
+                    double rmax, const float *disp_dev, void *rho_dev,
+                    void *p_dev, void *cs_dev);
+
+/* TaitEquation alone: p, cs from rho (eos_funcs.pyx:105-116). */
+int32_t sph_tait(void *stream, const sph_params *prm, const sph_view *a,
+                 void *p_dev, void *cs_dev);
+
+/* MomentumEquation (pressure_funcs.pyx:154-245) over sources flagged
+ * SPH_SRC_MOMENTUM and XSPHCorrection (xsph_funcs.pyx:31-88) over sources
+ * flagged SPH_SRC_XSPH, one pass.  acc_dev[3] / dv_dev[3] are the outputs
+ * (components >= dim may be NULL; a NULL acc_dev[0] or dv_dev[0] skips that
+ * operation).  GravityForce is NOT added here (it has its own k array). */
+int32_t sph_momentum_xsph(void *stream, const sph_grid *g, const sph_params *prm,
+                          const sph_view *dst, int32_t nsrc, const sph_view *srcs,
+                          const uint32_t *src_flags_host, double rmax,
+                          const float *disp_dev, void *const *acc_dev,
+                          void *const *dv_dev);
+
+/* Fused WCSPH integrator stage for one fluid destination array: the same
+ * pair sums as sph_momentum_xsph followed by the reference integrator's
+ * update of that particle (integrator.py:472-514, :534-551, :597-619), with
+ * PositionStepping (k_x = u, position_funcs.pyx:40-60) and GravityForce
+ * (k_u = g, external_force.pyx:39-45) as separate calcs in this order:
+ * mom, gravity, step, xsph.
+ *   stage 1 (RK2 predictor):  k1 = (a, dv) -> k1_dev[6];
+ *            out.u = (u0 + a*dt) + g*dt;  out.x = (x0 + u0*dt) + dv*dt
+ *   stage 2 (RK2 final):      out.u = (u0 + hdt*(k1a+a)) + hdt*(g+g)
+ *            out.x = (x0 + hdt*(u0+u)) + hdt*(k1dv+dv),  hdt = 0.5*dt
+ *   stage 0 (Euler):          out.u = (u0 + a*dt) + g*dt;
+ *            out.x = (x0 + u0*dt) + dv*dt
+ * `dst` is the state the right-hand side is evaluated at; state0_dev[6]
+ * holds x0,y0,z0,u0,v0,w0; out_dev[6] receives the new state (it may alias
+ * state0_dev only for stage 2, whose pair sums read `dst` and `srcs`, never
+ * state0).  disp_out_dev (nullable) receives the max |out.x - x0| component
+ * via atomic max (for the stage-2 ring), it must be zeroed by the caller. */
+int32_t sph_wcsph_stage(void *stream, int32_t stage, const sph_grid *g,
+                        const sph_params *prm, const sph_view *dst, int32_t nsrc,
+                        const sph_view *srcs, const uint32_t *src_flags_host,
+                        double rmax, const float *disp_dev, double dt,
+                        const void *const *state0_dev, void *const *k1_dev,
+                        void *const *out_dev, float *disp_out_dev);
+
+/* ---- integrator updates of the generic path (integrator.py:472-514,
+ *      :534-551, :597-619) --------------------------------------------- */
+
+/* mode 0: out = a + b*s                 (Integrator.step, integrator.py:505)
+ * mode 1: out = a + s*(b + c); a = out  (RK2 final_step, s = 0.5*dt, :616)
+ * mode 2: out = a + b*s; a = out        (Euler final_step, :550)
+ * mode 3: out = a                        (set_initial_arrays / reset copy,
+ *                                         PositionStepping k = u, :40-60)
+ * mode 4: out = s                        (GravityForce k = g, :39-45)
+ * With tag_dev non-NULL, modes 3 and 4 write 0 where tag != LocalReal
+ * (sph_func.pyx:211-234).  nops arrays of n elements each, one dtype. */
+typedef struct sph_update_op {
+    void *out;
+    void *a;
+    const void *b;
+    const void *c;
+} sph_update_op;
+int32_t sph_update(void *stream, int32_t dtype, int64_t n, int32_t mode, double s,
+                   int32_t nops, const sph_update_op *ops_host, const int64_t *tag_dev);
+
+/* FP64 conservation sums over LocalReal particles of one array, added to
+ * out_dev[8]: mass, momentum x,y,z, kinetic energy, -sum m g.x, Tait elastic
+ * energy sum m e(rho), total (no reference counterpart; the reference
+ * reports none).  The caller zeroes out_dev once per set of arrays. */
+int32_t sph_conservation_f64(void *stream, const sph_params *prm, const sph_view *a,
+                             double *out_dev);
+
+/* Fill n int32 words with a value (graph-capturable, no hipMemset). */
+int32_t sph_fill_i32(void *stream, int32_t *dst_dev, int64_t n, int32_t value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPH_MI355X_H */

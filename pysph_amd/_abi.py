@@ -1,0 +1,146 @@
+"""ctypes binding of libsph_mi355x.so (declared in include/sph_mi355x.h).
+
+The library is the only compute path of this package: every neighbour
+search, pair sum and integrator update runs in it on the GPU.  Loading fails
+loudly when it is missing; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsph_mi355x.so")
+
+SPH_OK = 0
+SPH_EINVAL = -1
+SPH_ECAPACITY = -2
+SPH_EJUMP = -3
+SPH_EUNSUPPORTED = -4
+SPH_F32 = 0
+SPH_F64 = 1
+SPH_MAX_SRC = 4
+SPH_SRC_MOMENTUM = 1
+SPH_SRC_XSPH = 2
+SPH_PAIR_NATIVE = 0
+SPH_PAIR_F64 = 1
+SPH_BOUNDS_WORDS = 456
+
+_vp = ctypes.c_void_p
+
+
+class SphView(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("dtype", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)] + [
+        (f, _vp) for f in ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs")] + [
+        ("tag", _vp), ("cell", _vp), ("cell_start", _vp)]
+
+
+class SphGrid(ctypes.Structure):
+    _fields_ = [("cell_size", ctypes.c_double), ("lo", ctypes.c_int32 * 3),
+                ("dims", ctypes.c_int32 * 3)]
+
+
+class SphParams(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("pair_prec", ctypes.c_int32),
+                ("radius_scale", ctypes.c_double), ("co", ctypes.c_double),
+                ("ro", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
+                ("eta", ctypes.c_double), ("eps", ctypes.c_double),
+                ("g", ctypes.c_double * 3), ("has_gravity", ctypes.c_int32),
+                ("has_xsph", ctypes.c_int32)]
+
+
+class SphUpdateOp(ctypes.Structure):
+    _fields_ = [("out", _vp), ("a", _vp), ("b", _vp), ("c", _vp)]
+
+
+class SphError(RuntimeError):
+    """A non-zero return code of the C ABI."""
+
+    def __init__(self, fn, rc, msg):
+        super().__init__("%s failed (rc=%d): %s" % (fn, rc, msg))
+        self.rc = rc
+
+
+class JumpToleranceError(SphError):
+    """A particle moved more than one cell width between two binnings
+    (the reference raises RuntimeError, cell.pyx:1422-1437)."""
+
+
+# name -> (restype, argtypes)
+_P = ctypes.POINTER
+_SIGS = {
+    "sph_abi_version": (ctypes.c_int32, []),
+    "sph_last_error": (ctypes.c_char_p, []),
+    "sph_device_count": (ctypes.c_int32, []),
+    "sph_bounds": (ctypes.c_int32, [_vp, _P(SphView), _vp]),
+    "sph_bin_workspace_bytes": (ctypes.c_size_t, [ctypes.c_int64, ctypes.c_int64]),
+    "sph_bin": (ctypes.c_int32, [_vp, _P(SphView), _P(SphGrid), _vp, _vp, _vp, _vp, _vp,
+                                 _vp, ctypes.c_size_t]),
+    "sph_permute": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, ctypes.c_int32, _P(_vp),
+                                     _P(_vp), _P(ctypes.c_int32)]),
+    "sph_neighbor_counts": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
+                                             ctypes.c_double, _vp]),
+    "sph_scan_counts": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, _vp,
+                                         _P(ctypes.c_int64)]),
+    "sph_neighbor_fill": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
+                                           ctypes.c_double, _vp, _vp, _vp, ctypes.c_int64,
+                                           ctypes.c_int64, _P(ctypes.c_int64)]),
+    "sph_density": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
+                                     ctypes.c_int32, _P(SphView), ctypes.c_double, _vp, _vp,
+                                     _vp, _vp]),
+    "sph_tait": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp, _vp]),
+    "sph_momentum_xsph": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
+                                           ctypes.c_int32, _P(SphView), _P(ctypes.c_uint32),
+                                           ctypes.c_double, _vp, _P(_vp), _P(_vp)]),
+    "sph_wcsph_stage": (ctypes.c_int32, [_vp, ctypes.c_int32, _P(SphGrid), _P(SphParams),
+                                         _P(SphView), ctypes.c_int32, _P(SphView),
+                                         _P(ctypes.c_uint32), ctypes.c_double, _vp,
+                                         ctypes.c_double, _P(_vp), _P(_vp), _P(_vp), _vp]),
+    "sph_update": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
+                                    ctypes.c_double, ctypes.c_int32, _P(SphUpdateOp), _vp]),
+    "sph_conservation_f64": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp]),
+    "sph_fill_i32": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32]),
+}
+
+EXPORTS = tuple(_SIGS)
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Load the library (no GPU call is made).  Raises OSError when the
+    library has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise OSError("libsph_mi355x.so is not built (%s); run "
+                      "`python -m pysph_amd.build` first" % path)
+    L = ctypes.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    if L.sph_abi_version() != 1:
+        raise OSError("libsph_mi355x.so ABI version mismatch")
+    _lib = L
+    return L
+
+
+def call(name, *args):
+    """Call an ABI entry point and raise SphError on a non-zero return."""
+    L = load()
+    rc = getattr(L, name)(*args)
+    if rc != SPH_OK:
+        msg = L.sph_last_error().decode(errors="replace")
+        cls = JumpToleranceError if rc == SPH_EJUMP else SphError
+        raise cls(name, rc, msg)
+    return rc
+
+
+def ptr_array(ptrs):
+    """A C array of void* from ints / None."""
+    arr = (_vp * max(len(ptrs), 1))()
+    for i, p in enumerate(ptrs):
+        arr[i] = p if p else None
+    return arr

@@ -1,0 +1,306 @@
+"""CellManager: GPU cell binning of all particle arrays of a simulation.
+
+Reference: base/cell.pyx.  The reference keeps a Python dict of Cell
+objects keyed by IntPoint(floor(x/cs)) and moves escaped particles between
+cells incrementally (cell.pyx:319-432, :927-1015).  Here every array is
+re-binned on the GPU by a stable counting sort on the linear cell key of a
+dense grid (csrc/sph_bin.hip) and its properties are permuted into cell
+order, so each stencil row of cells is one contiguous particle range.
+
+The cell of a particle is the reference's floor(x/cell_size)
+(cell.pyx:42-82); the grid offset `lo` is an integer, so the key is exact.
+The cell size follows compute_cell_size (cell.pyx:893-925).
+"""
+import math
+
+import numpy as np
+import torch
+
+from .. import _abi
+from .._device import require_gpu
+from .point import IntPoint
+
+
+def real_to_int(real_val, step):
+    """floor(real_val/step) as an int (cell.pyx:42-59)."""
+    return int(math.floor(real_val / step))
+
+
+def find_cell_id(pnt, cell_size):
+    """Cell index of a point (cell.pyx:61-82)."""
+    return IntPoint(real_to_int(pnt.x, cell_size), real_to_int(pnt.y, cell_size),
+                    real_to_int(pnt.z, cell_size))
+
+
+py_real_to_int = real_to_int
+py_find_cell_id = find_cell_id
+
+
+def construct_immediate_neighbor_list(cell_id, include_self=True, distance=1):
+    """Ids of the (2d+1)^3 cells around cell_id, x outer, z inner
+    (cell.pyx:93-116)."""
+    out = []
+    for i in range(-distance, distance + 1):
+        for j in range(-distance, distance + 1):
+            for k in range(-distance, distance + 1):
+                out.append(IntPoint(cell_id.x + i, cell_id.y + j, cell_id.z + k))
+    if not include_self:
+        n = len(out)
+        out[n // 2] = out[n - 1]
+        out.pop()
+    return out
+
+
+class CellManager:
+    """Bins every array of `arrays_to_bin` on one shared grid."""
+
+    def __init__(self, arrays_to_bin=(), min_cell_size=-1.0, max_cell_size=0,
+                 periodic_domain=None, initialize=True, max_radius_scale=2.0):
+        if periodic_domain is not None:
+            raise NotImplementedError("periodic domains are not supported")
+        self.arrays_to_bin = list(arrays_to_bin)
+        self.min_cell_size = min_cell_size
+        self.max_cell_size = max_cell_size
+        self.max_radius_scale = max_radius_scale
+        self.jump_tolerance = 1
+        # relative margin of the cell size over the search radius; a binning
+        # stays reusable while particles move less than skin*radius/2
+        self.skin = 0.0
+        # True: check at every binning (one read-back); "deferred": count on
+        # the device and check in verify_jumps(); False: no check
+        self.check_jumps = True
+        self.cell_size = 0.0
+        self.min_h = self.max_h = 0.0
+        self.is_dirty = True
+        self.initialized = False
+        self.grid = None          # _abi.SphGrid of the last binning
+        self.fixed_domain = None  # optional ((xmin,ymin,zmin),(xmax,ymax,zmax))
+        self.grid_slack = 2       # extra cells on every side of the bounds
+        self.coord_x, self.coord_y, self.coord_z = "x", "y", "z"
+        self.num_binnings = 0
+        self._ws = None
+        self._jump_acc = None
+        if initialize and self.arrays_to_bin:
+            self.initialize()
+
+    # ------------------------------------------------------------------
+    def _compute_minmax_h(self):
+        mn, mx = math.inf, 0.0
+        for pa in self.arrays_to_bin:
+            if pa.get_number_of_particles() == 0:
+                continue
+            h = pa.get_carray("h")
+            mn = min(mn, float(h.min()))
+            mx = max(mx, float(h.max()))
+        return (0.0 if mn == math.inf else mn), mx
+
+    def compute_cell_size(self, min_size=0, max_size=0):
+        """cell_size = max_radius_scale * max h unless min_size > 0, capped by
+        max_size, 1.0 when zero (cell.pyx:893-925)."""
+        self.min_h, self.max_h = self._compute_minmax_h()
+        if min_size <= 0:
+            self.cell_size = self.max_radius_scale * self.max_h * (1.0 + self.skin)
+        else:
+            self.cell_size = min_size
+        if max_size > 0 and self.cell_size > max_size:
+            self.cell_size = max_size
+        if self.cell_size == 0.0:
+            self.cell_size = 1.0
+        return self.cell_size
+
+    def initialize(self):
+        self.compute_cell_size(self.min_cell_size, self.max_cell_size)
+        self.initialized = True
+        self.is_dirty = True
+        for pa in self.arrays_to_bin:
+            pa.set_dirty(True)
+
+    def set_skin(self, skin):
+        """Cell size = max_radius_scale * max h * (1 + skin); re-bins."""
+        self.skin = float(skin)
+        if self.min_cell_size <= 0:
+            self.initialize()
+            self.grid = None
+
+    def set_fixed_domain(self, lo, hi):
+        """Bin on a fixed box (no bounds read-back): needed for graph capture.
+        Particles outside it are clamped into border cells, which keeps the
+        search complete."""
+        self.fixed_domain = (tuple(lo), tuple(hi))
+        self.grid = None
+        self.is_dirty = True
+
+    def update_status(self):
+        for pa in self.arrays_to_bin:
+            if pa.is_dirty:
+                self.is_dirty = True
+
+    # ------------------------------------------------------------------
+    def _bounds(self):
+        lo = np.full(3, np.inf)
+        hi = np.full(3, -np.inf)
+        hmax = 0.0
+        for pa in self.arrays_to_bin:
+            if pa.get_number_of_particles() == 0:
+                continue
+            stream = require_gpu(pa.get_carray("x"))
+            out = torch.empty(_abi.SPH_BOUNDS_WORDS, dtype=torch.float64, device=pa.device)
+            v = pa.view(("x", "y", "z", "h"))
+            _abi.call("sph_bounds", stream, v, out.data_ptr())
+            b = out[:7].cpu().numpy()
+            lo = np.minimum(lo, b[:3])
+            hi = np.maximum(hi, b[3:6])
+            hmax = max(hmax, float(b[6]))
+        if hmax > 0:
+            self.max_h = hmax
+        if not np.all(np.isfinite(lo)) or not np.all(np.isfinite(hi)):
+            bad = ~(np.isfinite(lo) & np.isfinite(hi))
+            lo[bad] = 0.0
+            hi[bad] = 0.0
+        return lo, hi
+
+    def make_grid(self, lo, hi):
+        cs = self.cell_size
+        g = _abi.SphGrid()
+        g.cell_size = cs
+        s = self.grid_slack
+        for d in range(3):
+            c0 = math.floor(lo[d] / cs) - s
+            c1 = math.floor(hi[d] / cs) + s
+            g.lo[d] = int(max(min(c0, 2**30), -2**30))
+            g.dims[d] = int(max(1, min(c1 - c0 + 1, 2**20)))
+        ncells = g.dims[0] * g.dims[1] * g.dims[2]
+        if ncells > 2**31 - 2:
+            raise MemoryError("cell grid of %d cells is too large" % ncells)
+        return g
+
+    @staticmethod
+    def grid_key(g):
+        return (g.cell_size, tuple(g.lo), tuple(g.dims))
+
+    def ncells(self):
+        g = self.grid
+        return g.dims[0] * g.dims[1] * g.dims[2]
+
+    def _workspace(self, n, ncells, device):
+        need = _abi.load().sph_bin_workspace_bytes(n, ncells)
+        if self._ws is None or self._ws.numel() < need or self._ws.device != device:
+            self._ws = torch.empty(need, dtype=torch.uint8, device=device)
+        return self._ws
+
+    def update(self):
+        """Re-bin dirty arrays (all arrays when the grid changes)."""
+        if not self.initialized:
+            self.initialize()
+        self.update_status()
+        if not self.is_dirty and self.grid is not None:
+            return 0
+        if self.fixed_domain is not None:
+            lo, hi = self.fixed_domain
+        else:
+            lo, hi = self._bounds()
+        g = self.make_grid(lo, hi)
+        if self.grid is not None and self.fixed_domain is None:
+            # keep the previous grid when it still covers the bounds, so that
+            # unchanged arrays keep valid bins and the jump check applies
+            old = self.grid
+            covers = old.cell_size == g.cell_size and all(
+                old.lo[d] <= g.lo[d] + self.grid_slack - 1 and
+                old.lo[d] + old.dims[d] >= g.lo[d] + g.dims[d] - self.grid_slack + 1
+                for d in range(3))
+            if covers:
+                g = old
+        key = self.grid_key(g)
+        for pa in self.arrays_to_bin:
+            if pa.is_dirty or pa._grid_key != key:
+                self.bin_array(pa, g)
+        self.grid = g
+        self.is_dirty = False
+        return 0
+
+    def bin_array(self, pa, g, pos=None, extra=()):
+        """Counting-sort pa by cell on grid g and permute its properties.
+        `pos` (three tensors) bins by other positions than x, y, z (a
+        predictor stage); `extra` per-particle tensors are permuted along and
+        returned."""
+        n = pa.get_number_of_particles()
+        key = self.grid_key(g)
+        ncells = g.dims[0] * g.dims[1] * g.dims[2]
+        dev = pa.device
+        cell_start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
+        if n == 0:
+            stream = require_gpu(cell_start)
+            _abi.call("sph_fill_i32", stream, cell_start.data_ptr(), ncells + 1, 0)
+            pa.__dict__["_cell"] = torch.empty(0, dtype=torch.int32, device=dev)
+            pa.__dict__["_cell_start"] = cell_start
+            pa.__dict__["_grid_key"] = key
+            pa.set_dirty(False)
+            return
+        stream = require_gpu(pa.get_carray("x"))
+        perm = torch.empty(n, dtype=torch.int32, device=dev)
+        cell = torch.empty(n, dtype=torch.int32, device=dev)
+        ws = self._workspace(n, ncells, dev)
+        prev = pa._cell if (self.check_jumps and pa._grid_key == key and
+                            pa._cell is not None and pa._cell.numel() == n) else None
+        jumps = None
+        if prev is not None:
+            if self.check_jumps == "deferred":
+                # accumulate on the device; verify_jumps() reads it later
+                if self._jump_acc is None or self._jump_acc.device != dev:
+                    self._jump_acc = torch.zeros(1, dtype=torch.int32, device=dev)
+                jumps = self._jump_acc
+            else:
+                jumps = torch.zeros(1, dtype=torch.int32, device=dev)
+        v = pa.view(("x", "y", "z", "h"))
+        if pos is not None:
+            v.x, v.y, v.z = (t.data_ptr() for t in pos)
+        _abi.call("sph_bin", stream, v, g, perm.data_ptr(), cell.data_ptr(),
+                  cell_start.data_ptr(), prev.data_ptr() if prev is not None else None,
+                  jumps.data_ptr() if jumps is not None else None, ws.data_ptr(), ws.numel())
+        if jumps is not None and self.check_jumps != "deferred":
+            nj = int(jumps.item())
+            if nj > 0:
+                raise _abi.JumpToleranceError(
+                    "CellManager.update", _abi.SPH_EJUMP,
+                    "%d particle(s) of %r moved by more than one cell width "
+                    "(jump tolerance %d)" % (nj, pa.name, self.jump_tolerance))
+        moved = pa.apply_permutation(perm, extra)
+        pa.__dict__["_cell"] = cell
+        pa.__dict__["_cell_start"] = cell_start
+        pa.__dict__["_grid_key"] = key
+        pa.set_dirty(False)
+        self.num_binnings += 1
+        return moved
+
+    def verify_jumps(self):
+        """Raise if any deferred jump check counted a particle that moved
+        more than one cell between two binnings."""
+        if self._jump_acc is not None:
+            nj = int(self._jump_acc.item())
+            if nj > 0:
+                raise _abi.JumpToleranceError(
+                    "CellManager.update", _abi.SPH_EJUMP,
+                    "%d particle move(s) of more than one cell width" % nj)
+        return 0
+
+    def get_number_of_particles(self):
+        return sum(pa.get_number_of_particles() for pa in self.arrays_to_bin)
+
+    def cells_of(self, pa):
+        """Host view of the binning of pa: dict IntPoint -> user indices (the
+        reference's cells_dict content for one array)."""
+        self.update()
+        g = self.grid
+        cs = pa._cell_start.cpu().numpy()
+        order = pa._order_np()
+        out = {}
+        for c in np.nonzero(np.diff(cs))[0]:
+            cx = c % g.dims[0]
+            t = c // g.dims[0]
+            cy = t % g.dims[1]
+            cz = t // g.dims[1]
+            ids = np.arange(cs[c], cs[c + 1])
+            if order is not None:
+                ids = order[ids]
+            out[IntPoint(cx + g.lo[0], cy + g.lo[1], cz + g.lo[2])] = np.sort(ids)
+        return out

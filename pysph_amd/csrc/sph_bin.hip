@@ -1,0 +1,588 @@
+// Cell binning (stable counting sort by cell key), the permutation gather,
+// bounds, scans and the exact neighbour-list export.
+//
+// Reference behaviour being replaced: CellManager.update / Cell.update bin
+// particles into a dict of cells keyed by floor(x/cell_size)
+// (cell.pyx:42-82, :319-432, :927-1015), and FixedDestNbrParticleLocator
+// caches, per destination, the source indices passing the strict FP64 test
+// (nnps.pyx:271-356, :525-620).
+#include <math.h>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+// ---------------------------------------------------------------------------
+// block / wave scan helpers (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+template <typename V>
+__device__ __forceinline__ V wave_inclusive_scan(V x) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    V y = __shfl_up(x, off, 64);
+    if (lane >= off) x += y;
+  }
+  return x;
+}
+
+// Exclusive scan over a block of kBlock threads; returns the thread's
+// exclusive prefix and the block total.
+template <typename V>
+__device__ __forceinline__ V block_exclusive_scan(V x, V& total) {
+  __shared__ V wsum[kBlock / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  V inc = wave_inclusive_scan(x);
+  if (lane == 63) wsum[wid] = inc;
+  __syncthreads();
+  V off = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; ++k) {
+    V s = wsum[k];
+    if (k < wid) off += s;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - x;
+}
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(kBlock) void scan_reduce(const In* __restrict__ in, int64_t n,
+                                                      Out* __restrict__ partial) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile;
+  Out s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + (int64_t)k * kBlock + threadIdx.x;
+    if (i < n) s += (Out)in[i];
+  }
+  Out tot;
+  block_exclusive_scan<Out>(s, tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// One block scans the partials in place (exclusive) and writes the grand
+// total to *total.
+template <typename Out>
+__global__ __launch_bounds__(kBlock) void scan_partials(Out* __restrict__ partial, int nb,
+                                                        Out* __restrict__ total) {
+  Out carry = 0;
+  for (int base = 0; base < nb; base += kBlock) {
+    int i = base + threadIdx.x;
+    Out x = i < nb ? partial[i] : 0;
+    Out tot;
+    Out ex = block_exclusive_scan<Out>(x, tot);
+    if (i < nb) partial[i] = ex + carry;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+
+template <typename In, typename Out>
+__global__ __launch_bounds__(kBlock) void scan_final(const In* __restrict__ in, int64_t n,
+                                                     const Out* __restrict__ partial,
+                                                     Out* __restrict__ out) {
+  const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+  Out v[kScanItems];
+  Out s = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    v[k] = i < n ? (Out)in[i] : 0;
+    s += v[k];
+  }
+  Out tot;
+  Out ex = block_exclusive_scan<Out>(s, tot) + partial[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kScanItems; ++k) {
+    int64_t i = base + k;
+    if (i < n) out[i] = ex;
+    ex += v[k];
+  }
+}
+
+// Exclusive scan of n counts into out[0..n]; out[n] = total.  `partial`
+// holds at least ceil(n / kScanTile) elements.
+template <typename In, typename Out>
+static int32_t exclusive_scan(hipStream_t st, const In* in, int64_t n, Out* out,
+                              Out* partial) {
+  const int nb = (int)((n + kScanTile - 1) / kScanTile);
+  if (nb > 0) {
+    hipLaunchKernelGGL((scan_reduce<In, Out>), dim3(nb), dim3(kBlock), 0, st, in, n,
+                       partial);
+  }
+  hipLaunchKernelGGL((scan_partials<Out>), dim3(1), dim3(kBlock), 0, st, partial, nb,
+                     out + n);
+  if (nb > 0) {
+    hipLaunchKernelGGL((scan_final<In, Out>), dim3(nb), dim3(kBlock), 0, st, in, n,
+                       partial, out);
+  }
+  return launch_status("exclusive_scan");
+}
+
+__global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+static int32_t fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+  if (n <= 0) return SPH_OK;
+  int nb = (int)((n + kBlock - 1) / kBlock);
+  if (nb > 65536) nb = 65536;
+  hipLaunchKernelGGL(fill_i32_kernel, dim3(nb), dim3(kBlock), 0, st, p, n, v);
+  return launch_status("fill_i32");
+}
+
+// ---------------------------------------------------------------------------
+// bounds
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bounds_partial(View<T> a, double* __restrict__ part) {
+  double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[4] = {-INFINITY, -INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    double c[3] = {(double)a.x[i], (double)a.y[i], (double)a.z[i]};
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      mn[d] = fmin(mn[d], c[d]);
+      mx[d] = fmax(mx[d], c[d]);
+    }
+    mx[3] = fmax(mx[3], (double)a.h[i]);
+  }
+  __shared__ double red[7][kBlock];
+  for (int d = 0; d < 3; ++d) {
+    red[d][threadIdx.x] = mn[d];
+    red[3 + d][threadIdx.x] = mx[d];
+  }
+  red[6][threadIdx.x] = mx[3];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int d = 0; d < 3; ++d)
+        red[d][threadIdx.x] = fmin(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+      for (int d = 3; d < 7; ++d)
+        red[d][threadIdx.x] = fmax(red[d][threadIdx.x], red[d][threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) part[blockIdx.x * 7 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void bounds_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  const int d = threadIdx.x;
+  if (d >= 7) return;
+  double v = d < 3 ? INFINITY : -INFINITY;
+  for (int b = 0; b < nb; ++b) v = d < 3 ? fmin(v, part[b * 7 + d]) : fmax(v, part[b * 7 + d]);
+  out[d] = v;
+}
+
+// ---------------------------------------------------------------------------
+// binning
+// ---------------------------------------------------------------------------
+struct BinGrid {
+  double cs;
+  int lo[3];
+  int dims[3];
+};
+
+// floor(x/cs) as the reference computes it (cell.pyx:57), shifted into the
+// grid and clamped.
+__device__ __forceinline__ int grid_coord(double x, double cs, int lo, int dim) {
+  const double f = floor(x / cs);
+  const double c = f - (double)lo;
+  if (!(c >= 0.0)) return 0;  // also catches NaN
+  if (c > (double)(dim - 1)) return dim - 1;
+  return (int)c;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
+                                                    int* __restrict__ cell_tmp,
+                                                    int* __restrict__ counts,
+                                                    const int* __restrict__ prev_cell,
+                                                    int* __restrict__ jumps) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  const int cx = grid_coord((double)a.x[i], g.cs, g.lo[0], g.dims[0]);
+  const int cy = grid_coord((double)a.y[i], g.cs, g.lo[1], g.dims[1]);
+  const int cz = grid_coord((double)a.z[i], g.cs, g.lo[2], g.dims[2]);
+  const int c = (cz * g.dims[1] + cy) * g.dims[0] + cx;
+  cell_tmp[i] = c;
+  atomicAdd(&counts[c], 1);
+  if (prev_cell != nullptr) {
+    const int pc = prev_cell[i];
+    const int px = pc % g.dims[0], t = pc / g.dims[0];
+    const int py = t % g.dims[1], pz = t / g.dims[1];
+    if (abs(px - cx) > 1 || abs(py - cy) > 1 || abs(pz - cz) > 1) atomicAdd(jumps, 1);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void bin_scatter(int n, const int* __restrict__ cell_tmp,
+                                                      const int* __restrict__ cstart,
+                                                      int* __restrict__ cursor,
+                                                      int* __restrict__ bucket) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_tmp[i];
+  const int pos = cstart[c] + atomicAdd(&cursor[c], 1);
+  bucket[pos] = i;
+}
+
+// The atomic scatter leaves each cell's members in arbitrary order; rank
+// every member by its current slot to get a deterministic, stable layout.
+__global__ __launch_bounds__(kBlock) void bin_rank(int n, const int* __restrict__ cell_tmp,
+                                                   const int* __restrict__ cstart,
+                                                   const int* __restrict__ bucket,
+                                                   int* __restrict__ perm,
+                                                   int* __restrict__ cell_out) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const int me = bucket[p];
+  const int c = cell_tmp[me];
+  const int s = cstart[c], e = cstart[c + 1];
+  int rank = 0;
+  for (int q = s; q < e; ++q) rank += bucket[q] < me ? 1 : 0;
+  perm[s + rank] = me;
+  cell_out[s + rank] = c;
+}
+
+// ---------------------------------------------------------------------------
+// permutation gather
+// ---------------------------------------------------------------------------
+constexpr int kPermFields = 16;
+struct PermArgs {
+  const void* src[kPermFields];
+  void* dst[kPermFields];
+  int esz[kPermFields];
+  int nf;
+};
+
+__global__ __launch_bounds__(kBlock) void permute_kernel(int n, const int* __restrict__ perm,
+                                                         PermArgs a) {
+  const int p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= n) return;
+  const int j = perm[p];
+  for (int f = 0; f < a.nf; ++f) {
+    if (a.esz[f] == 4)
+      ((uint32_t*)a.dst[f])[p] = ((const uint32_t*)a.src[f])[j];
+    else
+      ((uint64_t*)a.dst[f])[p] = ((const uint64_t*)a.src[f])[j];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exact neighbour lists
+// ---------------------------------------------------------------------------
+template <typename T>
+struct NbrArgs {
+  View<T> dst, src;
+  double rs;
+  const int64_t* ids;
+  const int64_t* off;
+  int64_t* out;
+  int64_t* counts;
+};
+
+template <typename T>
+__device__ __forceinline__ bool pair_within(T xs, T ys, T zs, T xd, T yd, T zd, float r2f,
+                                            double r2d);
+
+template <>
+__device__ __forceinline__ bool pair_within<float>(float xs, float ys, float zs, float xd,
+                                                   float yd, float zd, float r2f, double r2d) {
+  const float dx = xs - xd, dy = ys - yd, dz = zs - zd;
+  const float d2 = dx * dx + dy * dy + dz * dz;
+  return Within32::test(xs, ys, zs, xd, yd, zd, d2, r2f, r2d);
+}
+
+template <>
+__device__ __forceinline__ bool pair_within<double>(double xs, double ys, double zs, double xd,
+                                                    double yd, double zd, float, double r2d) {
+  return exact_within(xs, ys, zs, xd, yd, zd, r2d);
+}
+
+template <typename T, bool FILL>
+__global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int i = bid * kBlock + threadIdx.x;
+  if (i >= a.dst.n) return;
+  const T xd = a.dst.x[i], yd = a.dst.y[i], zd = a.dst.z[i];
+  const double r = (double)a.dst.h[i] * a.rs;  // eff_radius (nnps.pyx:552)
+  const double r2d = r * r;
+  const float r2f = (float)r2d;
+  int cx, cy, cz;
+  cell_xyz(a.dst.cell[i], g, cx, cy, cz);
+  // ring = ceil(r/cs) for this destination (cell.pyx:1127-1129)
+  const double rd = ceil(r / g.cs);
+  const int full = max(g.dims[0], max(g.dims[1], g.dims[2]));
+  int ring = !(rd < (double)full) ? full : (int)rd;
+  if (ring < 1) ring = 1;
+  int64_t k = FILL ? a.off[i] : 0;
+  const View<T>& S = a.src;
+  for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
+    for (int j = j0; j < j1; ++j) {
+      if (pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d)) {
+        if (FILL) a.out[k] = a.ids[j];
+        ++k;
+      }
+    }
+  });
+  if (!FILL) {
+    a.counts[i] = k;
+  } else {
+    // sort the row ascending (rows hold tens of entries)
+    const int64_t b = a.off[i];
+    for (int64_t p = b + 1; p < k; ++p) {
+      const int64_t v = a.out[p];
+      int64_t q = p - 1;
+      while (q >= b && a.out[q] > v) {
+        a.out[q + 1] = a.out[q];
+        --q;
+      }
+      a.out[q + 1] = v;
+    }
+  }
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+extern "C" {
+
+int32_t sph_fill_i32(void* stream, int32_t* dst_dev, int64_t n, int32_t value) {
+  if (n > 0 && dst_dev == nullptr) {
+    set_error("sph_fill_i32: NULL destination");
+    return SPH_EINVAL;
+  }
+  return fill_i32(as_stream(stream), dst_dev, n, value);
+}
+
+int32_t sph_bounds(void* stream, const sph_view* a, double* out_dev) {
+  SPH_TRY(check_view(a, "sph_bounds", false, false, false));
+  if (out_dev == nullptr) {
+    set_error("sph_bounds: NULL output");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  // partials live at the tail of the output buffer's companion: use a small
+  // fixed number of blocks and reduce them in a second single-block pass.
+  const int nb = 64;
+  static_assert(nb * 7 <= 4096, "partials");
+  double* part = nullptr;
+  // The 7-double output is followed by caller-provided room for partials.
+  part = out_dev + 8;
+  if (a->dtype == SPH_F32)
+    hipLaunchKernelGGL(bounds_partial<float>, dim3(nb), dim3(kBlock), 0, st, to_view<float>(*a),
+                       part);
+  else
+    hipLaunchKernelGGL(bounds_partial<double>, dim3(nb), dim3(kBlock), 0, st,
+                       to_view<double>(*a), part);
+  hipLaunchKernelGGL(bounds_final, dim3(1), dim3(64), 0, st, part, nb, out_dev);
+  return launch_status("sph_bounds");
+}
+
+size_t sph_bin_workspace_bytes(int64_t n, int64_t ncells) {
+  // counts/cursor (ncells), cell_tmp (n), bucket (n), scan partials
+  const int64_t nb = (ncells + 1 + kScanTile - 1) / kScanTile + 1;
+  size_t b = 0;
+  b += (size_t)(ncells + 64) * 4;
+  b += (size_t)(n + 64) * 4;
+  b += (size_t)(n + 64) * 4;
+  b += (size_t)(nb + 64) * 4;
+  return b + 1024;
+}
+
+int32_t sph_bin(void* stream, const sph_view* a, const sph_grid* g, int32_t* perm_dev,
+                int32_t* cell_dev, int32_t* cell_start_dev, const int32_t* prev_cell_dev,
+                int32_t* jump_count_dev, void* workspace_dev, size_t workspace_bytes) {
+  SPH_TRY(check_view(a, "sph_bin", false, false, false));
+  SPH_TRY(check_grid(g, "sph_bin"));
+  const int64_t n = a->n;
+  const int64_t ncells = (int64_t)g->dims[0] * g->dims[1] * g->dims[2];
+  if (workspace_bytes < sph_bin_workspace_bytes(n, ncells) || workspace_dev == nullptr) {
+    set_error("sph_bin: workspace of %zu bytes, %zu needed", workspace_bytes,
+              sph_bin_workspace_bytes(n, ncells));
+    return SPH_EINVAL;
+  }
+  if (!cell_start_dev || (n > 0 && (!perm_dev || !cell_dev))) {
+    set_error("sph_bin: NULL output");
+    return SPH_EINVAL;
+  }
+  if (prev_cell_dev != nullptr && jump_count_dev == nullptr) {
+    set_error("sph_bin: prev_cell given without jump_count");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  char* w = (char*)workspace_dev;
+  int* counts = (int*)w;
+  w += (size_t)(ncells + 64) * 4;
+  int* cell_tmp = (int*)w;
+  w += (size_t)(n + 64) * 4;
+  int* bucket = (int*)w;
+  w += (size_t)(n + 64) * 4;
+  int* partial = (int*)w;
+
+  BinGrid bg;
+  bg.cs = g->cell_size;
+  for (int d = 0; d < 3; ++d) {
+    bg.lo[d] = g->lo[d];
+    bg.dims[d] = g->dims[d];
+  }
+  SPH_TRY(fill_i32(st, counts, ncells, 0));
+  const int nb = nblocks_for(n);
+  if (n > 0) {
+    if (a->dtype == SPH_F32)
+      hipLaunchKernelGGL(bin_count<float>, dim3(nb), dim3(kBlock), 0, st, to_view<float>(*a),
+                         bg, cell_tmp, counts, prev_cell_dev, jump_count_dev);
+    else
+      hipLaunchKernelGGL(bin_count<double>, dim3(nb), dim3(kBlock), 0, st,
+                         to_view<double>(*a), bg, cell_tmp, counts, prev_cell_dev,
+                         jump_count_dev);
+    SPH_TRY(launch_status("sph_bin/count"));
+  }
+  SPH_TRY((exclusive_scan<int, int>(st, counts, ncells, cell_start_dev, partial)));
+  if (n == 0) return SPH_OK;
+  SPH_TRY(fill_i32(st, counts, ncells, 0));
+  hipLaunchKernelGGL(bin_scatter, dim3(nb), dim3(kBlock), 0, st, (int)n, cell_tmp,
+                     cell_start_dev, counts, bucket);
+  hipLaunchKernelGGL(bin_rank, dim3(nb), dim3(kBlock), 0, st, (int)n, cell_tmp,
+                     cell_start_dev, bucket, perm_dev, cell_dev);
+  return launch_status("sph_bin");
+}
+
+int32_t sph_permute(void* stream, int64_t n, const int32_t* perm_dev, int32_t nfields,
+                    const void* const* src_dev_host, void* const* dst_dev_host,
+                    const int32_t* elem_bytes_host) {
+  if (n < 0 || nfields < 0 || (n > 0 && nfields > 0 && (!perm_dev || !src_dev_host ||
+                                                          !dst_dev_host || !elem_bytes_host))) {
+    set_error("sph_permute: bad arguments");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  for (int f0 = 0; f0 < nfields; f0 += kPermFields) {
+    PermArgs pa;
+    pa.nf = nfields - f0 < kPermFields ? nfields - f0 : kPermFields;
+    for (int k = 0; k < pa.nf; ++k) {
+      const int f = f0 + k;
+      if ((elem_bytes_host[f] != 4 && elem_bytes_host[f] != 8) || !src_dev_host[f] ||
+          !dst_dev_host[f] || src_dev_host[f] == dst_dev_host[f]) {
+        set_error("sph_permute: field %d invalid (elem bytes %d, in-place not allowed)", f,
+                  elem_bytes_host[f]);
+        return SPH_EINVAL;
+      }
+      pa.src[k] = src_dev_host[f];
+      pa.dst[k] = dst_dev_host[f];
+      pa.esz[k] = elem_bytes_host[f];
+    }
+    if (n > 0)
+      hipLaunchKernelGGL(permute_kernel, dim3(nblocks_for(n)), dim3(kBlock), 0, st, (int)n,
+                         perm_dev, pa);
+  }
+  return launch_status("sph_permute");
+}
+
+static int32_t nbr_common(const sph_grid* g, const sph_view* dst, const sph_view* src,
+                          double rs, const char* who) {
+  SPH_TRY(check_grid(g, who));
+  SPH_TRY(check_view(dst, who, false, false, true));
+  SPH_TRY(check_view(src, who, false, false, true));
+  if (dst->dtype != src->dtype) {
+    set_error("%s: dst and src dtypes differ", who);
+    return SPH_EUNSUPPORTED;
+  }
+  if (!(rs > 0)) {
+    set_error("%s: radius_scale must be positive", who);
+    return SPH_EINVAL;
+  }
+  return SPH_OK;
+}
+
+int32_t sph_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst,
+                            const sph_view* src, double radius_scale, int64_t* counts_dev) {
+  SPH_TRY(nbr_common(g, dst, src, radius_scale, "sph_neighbor_counts"));
+  if (dst->n == 0) return SPH_OK;
+  if (!counts_dev) {
+    set_error("sph_neighbor_counts: NULL output");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  Grid G = make_grid(*g, 0.0);
+  const int nb = nblocks_for(dst->n);
+  if (dst->dtype == SPH_F32) {
+    NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, nullptr,
+                     nullptr, nullptr, counts_dev};
+    hipLaunchKernelGGL((nbr_kernel<float, false>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  } else {
+    NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, nullptr,
+                      nullptr, nullptr, counts_dev};
+    hipLaunchKernelGGL((nbr_kernel<double, false>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  }
+  return launch_status("sph_neighbor_counts");
+}
+
+int32_t sph_scan_counts(void* stream, int64_t n, const int64_t* counts_dev,
+                        int64_t* offsets_dev, int64_t* total_host) {
+  if (n < 0 || !offsets_dev || (n > 0 && !counts_dev)) {
+    set_error("sph_scan_counts: bad arguments");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  const int64_t nb = (n + kScanTile - 1) / kScanTile + 1;
+  int64_t* partial = nullptr;
+  hipError_t e = hipMallocAsync((void**)&partial, (size_t)nb * 8, st);
+  if (e != hipSuccess) {
+    set_error("sph_scan_counts: %s", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  int32_t rc = exclusive_scan<int64_t, int64_t>(st, counts_dev, n, offsets_dev, partial);
+  (void)hipFreeAsync(partial, st);
+  if (rc != SPH_OK) return rc;
+  if (total_host) {
+    e = hipMemcpyAsync(total_host, offsets_dev + n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error("sph_scan_counts: %s", hipGetErrorString(e));
+      return (int32_t)e;
+    }
+  }
+  return SPH_OK;
+}
+
+int32_t sph_neighbor_fill(void* stream, const sph_grid* g, const sph_view* dst,
+                          const sph_view* src, double radius_scale, const int64_t* src_ids_dev,
+                          const int64_t* offsets_dev, int64_t* nbrs_dev, int64_t cap,
+                          int64_t total, int64_t* needed_host) {
+  SPH_TRY(nbr_common(g, dst, src, radius_scale, "sph_neighbor_fill"));
+  if (needed_host) *needed_host = total;
+  if (cap < total) {
+    set_error("sph_neighbor_fill: capacity %lld < %lld pairs", (long long)cap,
+              (long long)total);
+    return SPH_ECAPACITY;
+  }
+  if (dst->n == 0 || total == 0) return SPH_OK;
+  if (!src_ids_dev || !offsets_dev || !nbrs_dev) {
+    set_error("sph_neighbor_fill: NULL argument");
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  Grid G = make_grid(*g, 0.0);
+  const int nb = nblocks_for(dst->n);
+  if (dst->dtype == SPH_F32) {
+    NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, src_ids_dev,
+                     offsets_dev, nbrs_dev, nullptr};
+    hipLaunchKernelGGL((nbr_kernel<float, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  } else {
+    NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, src_ids_dev,
+                      offsets_dev, nbrs_dev, nullptr};
+    hipLaunchKernelGGL((nbr_kernel<double, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  }
+  return launch_status("sph_neighbor_fill");
+}
+
+}  // extern "C"

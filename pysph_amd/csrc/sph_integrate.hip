@@ -1,0 +1,187 @@
+// Integrator array updates of the generic (per-calc) path and the FP64
+// conservation reductions.
+//
+// Reference: Integrator.step `current + k*dt` (integrator.py:505),
+// EulerIntegrator.final_step `p0 + k1*dt` (:550), RK2Integrator.final_step
+// `p0 + 0.5*dt*(k1 + k2)` (:616), set_initial_arrays / reset_current_arrays
+// copies (:339-392).  Each op of one launch updates one property array.
+#include <math.h>
+
+#include "sph_internal.h"
+
+namespace sph {
+
+constexpr int kMaxOps = 12;
+
+template <typename T>
+struct UpdArgs {
+  T* out[kMaxOps];
+  T* a[kMaxOps];
+  const T* b[kMaxOps];
+  const T* c[kMaxOps];
+  int nops;
+  int n;
+  T s;
+  const int64_t* tag;
+};
+
+template <typename T, int MODE>
+__global__ __launch_bounds__(kBlock) void update_kernel(UpdArgs<T> u) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= u.n) return;
+  for (int k = 0; k < u.nops; ++k) {
+    T r;
+    if (MODE == 0 || MODE == 2) {
+      r = u.a[k][i] + u.b[k][i] * u.s;
+    } else if (MODE == 1) {
+      r = u.a[k][i] + u.s * (u.b[k][i] + u.c[k][i]);
+    } else if (MODE == 3) {
+      r = (u.tag == nullptr || u.tag[i] == 0) ? u.a[k][i] : T(0);
+    } else {
+      r = (u.tag == nullptr || u.tag[i] == 0) ? u.s : T(0);
+    }
+    u.out[k][i] = r;
+    if (MODE == 1 || MODE == 2) u.a[k][i] = r;
+  }
+}
+
+template <typename T>
+static int32_t update_impl(hipStream_t st, int64_t n, int mode, double s, int nops,
+                           const sph_update_op* ops, const int64_t* tag) {
+  for (int k0 = 0; k0 < nops; k0 += kMaxOps) {
+    UpdArgs<T> u;
+    u.nops = nops - k0 < kMaxOps ? nops - k0 : kMaxOps;
+    u.n = (int)n;
+    u.s = (T)s;
+    u.tag = tag;
+    for (int k = 0; k < u.nops; ++k) {
+      const sph_update_op& op = ops[k0 + k];
+      if (!op.out || (mode != 4 && !op.a) || ((mode == 0 || mode == 1 || mode == 2) && !op.b) ||
+          (mode == 1 && !op.c)) {
+        set_error("sph_update: op %d has a NULL operand", k0 + k);
+        return SPH_EINVAL;
+      }
+      u.out[k] = (T*)op.out;
+      u.a[k] = (T*)op.a;
+      u.b[k] = (const T*)op.b;
+      u.c[k] = (const T*)op.c;
+    }
+    if (n == 0) continue;
+    const int nb = nblocks_for(n);
+    switch (mode) {
+      case 0:
+        hipLaunchKernelGGL((update_kernel<T, 0>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      case 1:
+        hipLaunchKernelGGL((update_kernel<T, 1>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      case 2:
+        hipLaunchKernelGGL((update_kernel<T, 2>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      case 3:
+        hipLaunchKernelGGL((update_kernel<T, 3>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      default:
+        hipLaunchKernelGGL((update_kernel<T, 4>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+    }
+  }
+  return launch_status("sph_update");
+}
+
+// ---------------------------------------------------------------------------
+// FP64 conservation sums: deterministic two-level reduction (per-block
+// partials in a fixed order, then one block).
+// ---------------------------------------------------------------------------
+constexpr int kConsBlocks = 512;
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void cons_partial(View<T> a, double g0, double g1,
+                                                       double g2, double ro, double gamma,
+                                                       double B, double* __restrict__ part) {
+  double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n;
+       i += (int64_t)gridDim.x * kBlock) {
+    if (a.tag != nullptr && a.tag[i] != 0) continue;
+    const double m = (double)a.m[i];
+    const double u = (double)a.u[i], v = (double)a.v[i], w = (double)a.w[i];
+    acc[0] += m;
+    acc[1] += m * u;
+    acc[2] += m * v;
+    acc[3] += m * w;
+    acc[4] += 0.5 * m * (u * u + v * v + w * w);
+    acc[5] -= m * (g0 * (double)a.x[i] + g1 * (double)a.y[i] + g2 * (double)a.z[i]);
+    const double rho = (double)a.rho[i];
+    const double e = B * (pow(rho / ro, gamma - 1.0) / ((gamma - 1.0) * ro) + 1.0 / rho -
+                          gamma / ((gamma - 1.0) * ro));
+    acc[6] += m * e;
+  }
+  __shared__ double red[7][kBlock];
+  for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = acc[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s)
+      for (int k = 0; k < 7; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) part[blockIdx.x * 8 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void cons_final(const double* __restrict__ part, int nb, double* __restrict__ out) {
+  if (threadIdx.x != 0) return;
+  for (int k = 0; k < 7; ++k) {
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += part[b * 8 + k];
+    out[k] += s;
+  }
+  out[7] = out[4] + out[5] + out[6];
+}
+
+}  // namespace sph
+
+using namespace sph;
+
+extern "C" {
+
+int32_t sph_update(void* stream, int32_t dtype, int64_t n, int32_t mode, double s, int32_t nops,
+                   const sph_update_op* ops_host, const int64_t* tag_dev) {
+  if (n < 0 || nops < 0 || (nops > 0 && !ops_host) || mode < 0 || mode > 4) {
+    set_error("sph_update: bad arguments");
+    return SPH_EINVAL;
+  }
+  if (dtype == SPH_F32) return update_impl<float>(as_stream(stream), n, mode, s, nops, ops_host, tag_dev);
+  if (dtype == SPH_F64) return update_impl<double>(as_stream(stream), n, mode, s, nops, ops_host, tag_dev);
+  set_error("sph_update: dtype unsupported");
+  return SPH_EUNSUPPORTED;
+}
+
+int32_t sph_conservation_f64(void* stream, const sph_params* prm, const sph_view* a,
+                             double* out_dev) {
+  if (prm == nullptr || out_dev == nullptr) {
+    set_error("sph_conservation_f64: bad arguments");
+    return SPH_EINVAL;
+  }
+  SPH_TRY(check_view(a, "sph_conservation_f64", true, true, false));
+  hipStream_t st = as_stream(stream);
+  if (a->n == 0) return SPH_OK;
+  double* part = nullptr;
+  hipError_t e = hipMallocAsync((void**)&part, (size_t)kConsBlocks * 8 * sizeof(double), st);
+  if (e != hipSuccess) {
+    set_error("sph_conservation_f64: %s", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  const double B = prm->co * prm->co * prm->ro / prm->gamma;
+  if (a->dtype == SPH_F32)
+    hipLaunchKernelGGL(cons_partial<float>, dim3(kConsBlocks), dim3(kBlock), 0, st,
+                       to_view<float>(*a), prm->g[0], prm->g[1], prm->g[2], prm->ro, prm->gamma,
+                       B, part);
+  else
+    hipLaunchKernelGGL(cons_partial<double>, dim3(kConsBlocks), dim3(kBlock), 0, st,
+                       to_view<double>(*a), prm->g[0], prm->g[1], prm->g[2], prm->ro, prm->gamma,
+                       B, part);
+  hipLaunchKernelGGL(cons_final, dim3(1), dim3(64), 0, st, part, kConsBlocks, out_dev);
+  (void)hipFreeAsync(part, st);
+  return launch_status("sph_conservation_f64");
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""Fused execution of the standard WCSPH calc set.
+
+Matched calc order (the order the reference's dam-break and elliptical-drop
+set-ups produce, examples/dam-break/dam_break.py:270-326 with summation
+density):
+
+    SPHRho       on any arrays (non-integrating, updates rho)
+    TaitEquation on arrays that have an SPHRho calc (updates p, cs)
+    MomentumEquation on the moving array F     (integrates u, v[, w])
+    GravityForce     on F, optional            (integrates u, v[, w])
+    PositionStepping on F                      (integrates x, y[, z])
+    XSPHCorrection   on F, optional            (integrates x, y[, z])
+
+Per RK2 step the plan issues, on one stream:
+
+    re-bin F (counting sort + permutation)          sph_bin / sph_permute
+    per array: density + Tait at X                  sph_density
+    momentum + XSPH + predictor update -> X1, k1    sph_wcsph_stage(1)
+    re-bin F by X1, permuting X, X1, k1 along       sph_bin / sph_permute
+    per array: density + Tait at X1                 sph_density
+    momentum + XSPH + final update -> X (in place)  sph_wcsph_stage(2)
+
+Stage 2 re-bins the moving array by the predicted positions X1 and carries
+X, X1 and k1 through the same permutation, as the reference rebuilds its
+neighbour caches after the predictor (integrator.py:621-645).  The update
+order inside the stage kernels is the reference's per-calc order
+(mom, gravity, step, xsph), so results match the generic path.
+"""
+import ctypes
+
+import torch
+
+from .. import _abi
+from .._device import ptr, require_gpu
+from ..sph import _ops
+from ..sph.funcs import (GravityForce, MomentumEquation, PositionStepping, SPHRho,
+                         TaitEquation, XSPHCorrection)
+
+
+class WCSPHPlan:
+    def __init__(self, particles, rho_calcs, tait, mom, grav, xsph, dim, radius_scale):
+        self.particles = particles
+        self.cm = particles.cell_manager
+        self.rho_calcs = rho_calcs
+        self.tait = tait            # TaitEquation function instance or None
+        self.tait_arrays = set()
+        self.mom = mom
+        self.fluid = mom.dest
+        self.grav = grav
+        self.xsph = xsph
+        self.dim = dim
+        self.radius_scale = radius_scale
+        f = mom.funcs[0]
+        g = (grav.funcs[0].gx, grav.funcs[0].gy, grav.funcs[0].gz) if grav else (0, 0, 0)
+        t = tait
+        self.prm = _ops.make_params(
+            dim, radius_scale=radius_scale,
+            co=t.co if t else 1.0, ro=t.ro if t else 1000.0, gamma=t.gamma if t else 7.0,
+            alpha=f.alpha, beta=f.beta, eta=f.eta,
+            eps=xsph.funcs[0].eps if xsph else 0.0, g=g, has_gravity=grav is not None,
+            has_xsph=xsph is not None)
+        # sources of the fused momentum pass: union of momentum and XSPH
+        # sources in particle-array order, with their roles
+        srcs, flags = [], []
+        for pa in particles.arrays:
+            fl = 0
+            if any(s is pa for s in mom.sources):
+                fl |= _abi.SPH_SRC_MOMENTUM
+            if xsph is not None and any(s is pa for s in xsph.sources):
+                fl |= _abi.SPH_SRC_XSPH
+            if fl:
+                srcs.append(pa)
+                flags.append(fl)
+        self.srcs = srcs
+        self.flags = (ctypes.c_uint32 * len(flags))(*flags)
+        self._bufs = None
+        self.steps = 0
+
+    # ----------------------------------------------------------------- match
+    @classmethod
+    def match(cls, particles, calcs):
+        def kind(c):
+            return type(c.funcs[0]) if c.funcs else None
+
+        i = 0
+        n = len(calcs)
+        rho_calcs = []
+        while i < n and kind(calcs[i]) is SPHRho and not calcs[i].integrates:
+            rho_calcs.append(calcs[i])
+            i += 1
+        tait_calcs = []
+        while i < n and kind(calcs[i]) is TaitEquation and not calcs[i].integrates:
+            tait_calcs.append(calcs[i])
+            i += 1
+
+        def take(k, integ=True):
+            nonlocal i
+            if i < n and kind(calcs[i]) is k and calcs[i].integrates == integ:
+                i += 1
+                return calcs[i - 1]
+            return None
+
+        mom = take(MomentumEquation)
+        grav = take(GravityForce)
+        step = take(PositionStepping)
+        xsph = take(XSPHCorrection)
+        if i != n or mom is None or step is None or not rho_calcs:
+            return None
+        fluid = mom.dest
+        dim = mom.kernel.dim
+        vel = ["u", "v", "w"][:dim]
+        pos = ["x", "y", "z"][:dim]
+        if mom.updates != vel or step.updates != pos or step.dest is not fluid:
+            return None
+        if grav is not None and (grav.dest is not fluid or grav.updates != vel):
+            return None
+        if xsph is not None and (xsph.dest is not fluid or xsph.updates != pos):
+            return None
+        if any(c.updates != ["rho"] for c in rho_calcs):
+            return None
+        rho_dests = [c.dest for c in rho_calcs]
+        if len({id(d) for d in rho_dests}) != len(rho_dests):
+            return None
+        tait_f = None
+        tait_arrays = []
+        for c in tait_calcs:
+            f = c.funcs[0]
+            if c.updates != ["p", "cs"] or not any(d is c.dest for d in rho_dests):
+                return None
+            if tait_f is not None and (f.co, f.ro, f.gamma) != (tait_f.co, tait_f.ro,
+                                                               tait_f.gamma):
+                return None
+            tait_f = f
+            tait_arrays.append(c.dest)
+        if len({pa.dtype for pa in particles.arrays}) != 1:
+            return None
+        kernel = mom.kernel
+        if rho_calcs[0].kernel.radius() != kernel.radius():
+            return None
+        if any(len(c.sources) > _abi.SPH_MAX_SRC for c in rho_calcs + [mom]):
+            return None
+        plan = cls(particles, rho_calcs, tait_f, mom, grav, xsph, dim, kernel.radius())
+        plan.tait_arrays = {id(a) for a in tait_arrays}
+        if len(plan.srcs) > _abi.SPH_MAX_SRC:
+            return None
+        return plan
+
+    # --------------------------------------------------------------- buffers
+    def _buffers(self):
+        pa = self.fluid
+        n = pa.get_number_of_particles()
+        if self._bufs is None or self._bufs["n"] != n or self._bufs["dtype"] != pa.dtype:
+            mk = lambda: torch.empty(n, dtype=pa.dtype, device=pa.device)  # noqa: E731
+            self._bufs = {
+                "n": n, "dtype": pa.dtype,
+                "s1": [mk() for _ in range(6)],
+                "k1": [mk() for _ in range(6)],
+                "disp": torch.zeros(1, dtype=torch.float32, device=pa.device),
+            }
+        return self._bufs
+
+    def _state0(self):
+        pa = self.fluid
+        return [pa.get_carray(f) for f in ("x", "y", "z", "u", "v", "w")]
+
+    def _s1_override(self, b):
+        """Fields of the stage state X1; components >= dim stay those of X."""
+        s0 = self._state0()
+        names = ("x", "y", "z", "u", "v", "w")
+        ov = {}
+        for c in range(6):
+            comp = c % 3
+            ov[names[c]] = b["s1"][c] if comp < self.dim else s0[c]
+        return {id(self.fluid): ov}
+
+    def _density(self, disp=None, override=None):
+        prm = self.prm
+        for c in self.rho_calcs:
+            pa = c.dest
+            rho = pa.get_carray("rho")
+            if id(pa) in self.tait_arrays:
+                _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
+                             pa.get_carray("cs"), disp=disp, override=override)
+            else:
+                _ops.density(self.cm, prm, pa, c.sources, rho, disp=disp, override=override)
+
+    def _stage(self, stage, dt, b, override=None, disp_in=None, disp_out=None):
+        pa = self.fluid
+        stream = require_gpu(pa.get_carray("x"))
+        vd = _ops.views([pa], override)
+        vs = _ops.views(self.srcs, override)
+        s0 = _abi.ptr_array([ptr(t) for t in self._state0()])
+        k1 = _abi.ptr_array([ptr(t) for t in b["k1"]])
+        if stage == 2:
+            out = s0
+        else:
+            out = _abi.ptr_array([ptr(t) for t in b["s1"]])
+        _abi.call("sph_wcsph_stage", stream, stage, self.cm.grid, self.prm, vd, len(self.srcs),
+                  vs, self.flags, _ops.search_radius(self.cm, self.radius_scale), ptr(disp_in),
+                  float(dt), s0, k1, out, ptr(disp_out))
+
+    # ----------------------------------------------------------------- steps
+    def rebin_stage(self, b):
+        """Re-bin the moving array by X1; X, X1 and k1 follow the new order."""
+        s0 = self._state0()
+        # components >= dim are not integrated: X1 keeps those of X
+        pos = [b["s1"][c] if c < self.dim else s0[c] for c in range(3)]
+        moved = self.cm.bin_array(self.fluid, self.cm.grid, pos=pos,
+                                  extra=b["s1"] + b["k1"])
+        b["s1"] = moved[:6]
+        b["k1"] = moved[6:]
+
+    def rk2_step(self, dt):
+        self.cm.update()
+        b = self._buffers()
+        self._density()
+        self._stage(1, dt, b)
+        self.rebin_stage(b)
+        ov = self._s1_override(b)
+        self._density(override=ov)
+        self._stage(2, dt, b, override=ov)
+        self.fluid.set_dirty(True)
+        self.steps += 1
+
+    def euler_step(self, dt):
+        self.cm.update()
+        b = self._buffers()
+        self._density()
+        self._stage(0, dt, b)
+        pa = self.fluid
+        names = ("x", "y", "z", "u", "v", "w")
+        for c in range(6):
+            if c % 3 < self.dim:
+                nm = names[c]
+                old = pa._props[nm]
+                pa._props[nm] = b["s1"][c]
+                b["s1"][c] = old
+        pa.set_dirty(True)
+        self.steps += 1

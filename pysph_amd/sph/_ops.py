@@ -1,0 +1,119 @@
+"""Launchers of the GPU operations behind the SPH function classes.
+
+Each launcher takes ParticleArrays (binned by their CellManager) and device
+output tensors, fills the ABI structs and calls libsph_mi355x.so on the
+current HIP stream.
+"""
+import ctypes
+
+import torch
+
+from .. import _abi
+from .._device import dtype_code, ptr, require_gpu
+
+# FP32 storage: the pair arithmetic of the neighbour passes is done in FP64
+# by default (SPH_PAIR_F64) so that pressure-gradient sums, which cancel
+# strongly in a fluid at rest, keep a relative error near 1e-7.  Set to
+# SPH_PAIR_NATIVE for all-FP32 pair arithmetic.
+DEFAULT_PAIR_PREC = _abi.SPH_PAIR_F64
+
+
+def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, beta=1.0,
+                eta=0.1, eps=0.5, g=(0.0, 0.0, 0.0), has_gravity=False, has_xsph=False,
+                pair_prec=None):
+    p = _abi.SphParams()
+    p.dim = int(dim)
+    p.pair_prec = DEFAULT_PAIR_PREC if pair_prec is None else int(pair_prec)
+    p.radius_scale = float(radius_scale)
+    p.co, p.ro, p.gamma = float(co), float(ro), float(gamma)
+    p.alpha, p.beta, p.eta, p.eps = float(alpha), float(beta), float(eta), float(eps)
+    for i in range(3):
+        p.g[i] = float(g[i])
+    p.has_gravity = 1 if has_gravity else 0
+    p.has_xsph = 1 if has_xsph else 0
+    return p
+
+
+def views(arrays, override=None):
+    """A C array of sph_view for the given ParticleArrays.  `override` maps
+    an array to a dict of field -> tensor replacing its own fields."""
+    override = override or {}
+    arr = (_abi.SphView * max(len(arrays), 1))()
+    for i, pa in enumerate(arrays):
+        v = pa.view()
+        for f, t in override.get(id(pa), {}).items():
+            setattr(v, f, ptr(t))
+        arr[i] = v
+    return arr
+
+
+def search_radius(cm, radius_scale):
+    """Largest search radius of a call: radius_scale * max h."""
+    return float(radius_scale) * float(cm.max_h if cm.max_h > 0 else cm.cell_size / 2.0)
+
+
+def check_same_dtype(arrays):
+    dts = {pa.dtype for pa in arrays}
+    if len(dts) != 1:
+        raise TypeError("all arrays of one operation must share a dtype")
+
+
+def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None):
+    """SPHRho over srcs into rho (+ Tait into p, cs)."""
+    check_same_dtype([dst] + list(srcs))
+    stream = require_gpu(rho)
+    vd = views([dst], override)
+    vs = views(srcs, override)
+    _abi.call("sph_density", stream, cm.grid, prm, vd, len(srcs), vs,
+              search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs))
+
+
+def tait(prm, dst, p, cs, rho=None):
+    stream = require_gpu(p)
+    v = dst.view()
+    if rho is not None:
+        v.rho = rho.data_ptr()
+    _abi.call("sph_tait", stream, prm, v, ptr(p), ptr(cs))
+
+
+def momentum_xsph(cm, prm, dst, srcs, flags, acc=None, dv=None, disp=None, override=None):
+    """MomentumEquation (sources flagged SPH_SRC_MOMENTUM) and XSPHCorrection
+    (flagged SPH_SRC_XSPH) into acc[3] / dv[3]."""
+    check_same_dtype([dst] + list(srcs))
+    stream = require_gpu(dst.get_carray("x"))
+    vd = views([dst], override)
+    vs = views(srcs, override)
+    fl = (ctypes.c_uint32 * max(len(srcs), 1))(*flags)
+    a = _abi.ptr_array([ptr(t) for t in acc]) if acc is not None else None
+    d = _abi.ptr_array([ptr(t) for t in dv]) if dv is not None else None
+    _abi.call("sph_momentum_xsph", stream, cm.grid, prm, vd, len(srcs), vs, fl,
+              search_radius(cm, prm.radius_scale), ptr(disp), a, d)
+
+
+def update(dtype, n, mode, s, ops, tag=None):
+    """Batched array updates (see sph_update in the header)."""
+    if not ops:
+        return
+    stream = require_gpu(*[o[0] for o in ops])
+    arr = (_abi.SphUpdateOp * len(ops))()
+    for i, (out, a, b, c) in enumerate(ops):
+        arr[i].out = ptr(out)
+        arr[i].a = ptr(a)
+        arr[i].b = ptr(b)
+        arr[i].c = ptr(c)
+    _abi.call("sph_update", stream, dtype_code(dtype), int(n), int(mode), float(s), len(ops),
+              arr, ptr(tag))
+
+
+def conservation(prm, arrays):
+    """FP64 mass, momentum, kinetic, potential, elastic and total energy
+    summed over the arrays (host numpy[8])."""
+    out = None
+    for pa in arrays:
+        if pa.get_number_of_particles() == 0:
+            continue
+        stream = require_gpu(pa.get_carray("x"))
+        if out is None:
+            out = torch.zeros(8, dtype=torch.float64, device=pa.device)
+        _abi.call("sph_conservation_f64", stream, prm, pa.view(), out.data_ptr())
+    return out.cpu().numpy() if out is not None else torch.zeros(8).numpy()
