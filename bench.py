@@ -29,12 +29,18 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scale", type=float, default=1.0,
                     help="shrink the lattice (for quick checks); 1.0 = the named workload")
-    ap.add_argument("--pair-prec", choices=["f64", "native"], default="f64")
+    ap.add_argument("--pair-prec", choices=["f64", "native"], default="native")
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--cells-per-radius", type=int, default=1,
+                    help="cell size = 2h / k (neighbour sets are unchanged)")
+    ap.add_argument("--no-packed", action="store_true",
+                    help="use the SoA kernels instead of the packed fast path")
+    ap.add_argument("--no-list", action="store_true",
+                    help="momentum pass searches cells instead of reusing the density list")
     return ap.parse_args()
 
 
-def build_case(scale=1.0, precision="single"):
+def build_case(scale=1.0, precision="single", cells_per_radius=1, packed=True, nlist=True):
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
     from pysph_amd.solver import generators as gen
@@ -51,6 +57,12 @@ def build_case(scale=1.0, precision="single"):
                             gravity=info["g"], xsph_eps=info["eps"])
     s.setup_integrator(particles)
     s.set_time_step(info["dt"])
+    plan = s.integrator.plan
+    if plan is not None:
+        if cells_per_radius != 1:
+            plan.set_cells_per_radius(cells_per_radius)
+        plan.use_packed = plan.use_packed and packed
+        plan.use_list = plan.use_list and plan.use_packed and nlist
     return s, particles, f, b, info, col, tank
 
 
@@ -173,7 +185,10 @@ def main():
     from pysph_amd.sph import _ops
     from pysph_amd import _abi
     _ops.DEFAULT_PAIR_PREC = _abi.SPH_PAIR_F64 if args.pair_prec == "f64" else _abi.SPH_PAIR_NATIVE
-    s, particles, f, b, info, col, tank = build_case(args.scale)
+    s, particles, f, b, info, col, tank = build_case(args.scale,
+                                                     cells_per_radius=args.cells_per_radius,
+                                                     packed=not args.no_packed,
+                                                     nlist=not args.no_list)
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, _ = pair_counts(particles, f, b)
@@ -221,6 +236,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "3D dam break, RK2, FP32 storage (pair arithmetic %s)"
                                % args.pair_prec,
+                   "cells_per_radius": args.cells_per_radius,
+                   "packed": not args.no_packed, "neighbour_list": not args.no_list,
                    "fluid_lattice": list(col), "tank_cells": list(tank),
                    "n_fluid": nf, "n_boundary": nb, "dx": info["dx"], "h": info["h"],
                    "dt": info["dt"], "c0": info["co"], "pairs_per_step": pairs_per_step,

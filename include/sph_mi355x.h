@@ -64,6 +64,12 @@ typedef struct sph_view {
     const int64_t *tag;        /* NULL: every particle LocalReal */
     const int32_t *cell;       /* per particle: linear cell from sph_bin */
     const int32_t *cell_start; /* ncells + 1 offsets from sph_bin */
+    /* Optional packed FP32 copies (float4 per particle) read by the fast
+     * neighbour passes: pos4 = (x,y,z,h), vel4 = (u,v,w,m) from sph_pack,
+     * aux4 = (rho, p, cs, p/rho^2) written by sph_density.  When pos4 and
+     * vel4 are set on the destination and every source (FP32 storage), the
+     * passes read them instead of the SoA fields. */
+    const void *pos4, *vel4, *aux4;
 } sph_view;
 
 /* The cell grid shared by all arrays of one Particles set.  Cell (i,j,k) of
@@ -129,6 +135,10 @@ int32_t sph_permute(void *stream, int64_t n, const int32_t *perm_dev,
                     int32_t nfields, const void *const *src_dev_host,
                     void *const *dst_dev_host, const int32_t *elem_bytes_host);
 
+/* Write the packed copies pos4[i] = (x,y,z,h) and vel4[i] = (u,v,w,m) of an
+ * FP32 array (float4 each; either may be NULL). */
+int32_t sph_pack(void *stream, const sph_view *a, void *pos4_dev, void *vel4_dev);
+
 /* ---- FixedDestNbrParticleLocator.get_nearest_particles (nnps.pyx:458-620)
  *      as a CSR export: the exact FP64 test
  *      ((xs-xd)^2 + (ys-yd)^2) + (zs-zd)^2 < (h_d*radius_scale)^2
@@ -152,16 +162,32 @@ int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
 
 /* ---- neighbour passes -------------------------------------------------- */
 
+/* Optional neighbour list handed from a density pass to the momentum pass
+ * of the same stage (same destination, same sources in the same order,
+ * unchanged positions).  Entry k of destination i is idx[k*n + i] (n = the
+ * destination count), entries grouped by source; count[s*n + i] is the
+ * number of accepted neighbours of i in source s.  When a destination has
+ * more than max_per neighbours in total, its entries stop at max_per and
+ * the momentum pass searches the cells for that destination instead.
+ * Only the packed fast path writes and reads the list. */
+typedef struct sph_nlist {
+    int32_t *idx;
+    int32_t *count;
+    int32_t max_per;
+    int32_t reserved;
+} sph_nlist;
+
 /* SPHRho over all sources (density_funcs.pyx:38-84), optionally followed by
  * TaitEquation on the destination (eos_funcs.pyx:105-116) when p_dev and
- * cs_dev are both non-NULL.  `disp_dev` (nullable, one float) is the largest
+ * cs_dev are both non-NULL.  With packed inputs the destination's aux4
+ * (when set) receives (rho, p, cs, p/rho^2).  `disp_dev` (nullable, one float) is the largest
  * coordinate displacement of any particle since the arrays were binned; the
  * stencil ring grows to cover it so that a binning can be reused across an
  * RK2 stage.  rmax = the largest search radius (radius_scale * max h). */
 int32_t sph_density(void *stream, const sph_grid *g, const sph_params *prm,
                     const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                     double rmax, const float *disp_dev, void *rho_dev,
-                    void *p_dev, void *cs_dev);
+                    void *p_dev, void *cs_dev, const sph_nlist *nlist_out);
 
 /* TaitEquation alone: p, cs from rho (eos_funcs.pyx:105-116). */
 int32_t sph_tait(void *stream, const sph_params *prm, const sph_view *a,
@@ -176,7 +202,7 @@ int32_t sph_momentum_xsph(void *stream, const sph_grid *g, const sph_params *prm
                           const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                           const uint32_t *src_flags_host, double rmax,
                           const float *disp_dev, void *const *acc_dev,
-                          void *const *dv_dev);
+                          void *const *dv_dev, const sph_nlist *nlist_in);
 
 /* Fused WCSPH integrator stage for one fluid destination array: the same
  * pair sums as sph_momentum_xsph followed by the reference integrator's
@@ -200,7 +226,8 @@ int32_t sph_wcsph_stage(void *stream, int32_t stage, const sph_grid *g,
                         const sph_view *srcs, const uint32_t *src_flags_host,
                         double rmax, const float *disp_dev, double dt,
                         const void *const *state0_dev, void *const *k1_dev,
-                        void *const *out_dev, float *disp_out_dev);
+                        void *const *out_dev, float *disp_out_dev,
+                        const sph_nlist *nlist_in);
 
 /* ---- integrator updates of the generic path (integrator.py:472-514,
  *      :534-551, :597-619) --------------------------------------------- */

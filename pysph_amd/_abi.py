@@ -31,7 +31,8 @@ class SphView(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("dtype", ctypes.c_int32),
                 ("reserved", ctypes.c_int32)] + [
         (f, _vp) for f in ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs")] + [
-        ("tag", _vp), ("cell", _vp), ("cell_start", _vp)]
+        ("tag", _vp), ("cell", _vp), ("cell_start", _vp), ("pos4", _vp), ("vel4", _vp),
+        ("aux4", _vp)]
 
 
 class SphGrid(ctypes.Structure):
@@ -47,6 +48,11 @@ class SphParams(ctypes.Structure):
                 ("eta", ctypes.c_double), ("eps", ctypes.c_double),
                 ("g", ctypes.c_double * 3), ("has_gravity", ctypes.c_int32),
                 ("has_xsph", ctypes.c_int32)]
+
+
+class SphNlist(ctypes.Structure):
+    _fields_ = [("idx", _vp), ("count", _vp), ("max_per", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class SphUpdateOp(ctypes.Structure):
@@ -87,19 +93,22 @@ _SIGS = {
                                            ctypes.c_int64, _P(ctypes.c_int64)]),
     "sph_density": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                      ctypes.c_int32, _P(SphView), ctypes.c_double, _vp, _vp,
-                                     _vp, _vp]),
+                                     _vp, _vp, _P(SphNlist)]),
     "sph_tait": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp, _vp]),
     "sph_momentum_xsph": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                            ctypes.c_int32, _P(SphView), _P(ctypes.c_uint32),
-                                           ctypes.c_double, _vp, _P(_vp), _P(_vp)]),
+                                           ctypes.c_double, _vp, _P(_vp), _P(_vp),
+                                           _P(SphNlist)]),
     "sph_wcsph_stage": (ctypes.c_int32, [_vp, ctypes.c_int32, _P(SphGrid), _P(SphParams),
                                          _P(SphView), ctypes.c_int32, _P(SphView),
                                          _P(ctypes.c_uint32), ctypes.c_double, _vp,
-                                         ctypes.c_double, _P(_vp), _P(_vp), _P(_vp), _vp]),
+                                         ctypes.c_double, _P(_vp), _P(_vp), _P(_vp), _vp,
+                                         _P(SphNlist)]),
     "sph_update": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                     ctypes.c_double, ctypes.c_int32, _P(SphUpdateOp), _vp]),
     "sph_conservation_f64": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp]),
     "sph_fill_i32": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32]),
+    "sph_pack": (ctypes.c_int32, [_vp, _P(SphView), _vp, _vp]),
 }
 
 EXPORTS = tuple(_SIGS)

@@ -268,6 +268,7 @@ class CellManager:
         pa.__dict__["_cell"] = cell
         pa.__dict__["_cell_start"] = cell_start
         pa.__dict__["_grid_key"] = key
+        pa.__dict__["_bin_version"] = pa.__dict__.get("_bin_version", 0) + 1
         pa.set_dirty(False)
         self.num_binnings += 1
         return moved

@@ -145,6 +145,9 @@ struct View {
   const int64_t* tag;
   const int* cell;
   const int* cstart;
+  const float4* pos4;  // packed (x,y,z,h), FP32 storage only
+  const float4* vel4;  // packed (u,v,w,m)
+  float4* aux4;        // packed (rho,p,cs,p/rho^2)
 };
 
 template <typename T>
@@ -156,6 +159,9 @@ inline View<T> to_view(const sph_view& a) {
   v.h = (const T*)a.h; v.m = (const T*)a.m; v.rho = (const T*)a.rho;
   v.p = (const T*)a.p; v.cs = (const T*)a.cs;
   v.tag = a.tag; v.cell = a.cell; v.cstart = a.cell_start;
+  v.pos4 = (const float4*)a.pos4;
+  v.vel4 = (const float4*)a.vel4;
+  v.aux4 = (float4*)a.aux4;
   return v;
 }
 

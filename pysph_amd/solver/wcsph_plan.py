@@ -75,6 +75,25 @@ class WCSPHPlan:
         self.flags = (ctypes.c_uint32 * len(flags))(*flags)
         self._bufs = None
         self.steps = 0
+        self._pk = {}
+        self._nl = None
+        self.use_list = False
+        self.list_ok = False
+        self.list_max_per = 128
+        # the packed fast path serves FP32 storage
+        self.use_packed = self.fluid.dtype == torch.float32
+
+    def set_cells_per_radius(self, k):
+        """Cell size = search radius / k (k = 1 is the reference's 2 max h).
+        Smaller cells test fewer candidates per neighbour at the price of
+        more stencil rows (ring k).  Neighbour sets do not change."""
+        cm = self.cm
+        if cm.max_h <= 0:
+            cm.compute_cell_size(cm.min_cell_size, cm.max_cell_size)
+        cm.min_cell_size = cm.max_radius_scale * cm.max_h / float(k)
+        cm.initialize()
+        cm.grid = None
+        self.cells_per_radius = k
 
     # ----------------------------------------------------------------- match
     @classmethod
@@ -141,6 +160,17 @@ class WCSPHPlan:
             return None
         plan = cls(particles, rho_calcs, tait_f, mom, grav, xsph, dim, kernel.radius())
         plan.tait_arrays = {id(a) for a in tait_arrays}
+        # packed (rho, p, cs) copies come from the density + Tait pass of
+        # each array; every momentum source needs one
+        plan.use_packed = plan.use_packed and all(id(pa) in plan.tait_arrays
+                                                  for pa in plan.srcs)
+        # the moving array's density pass can hand its neighbours to the
+        # momentum pass when both see the same sources in the same order
+        frho = [c for c in rho_calcs if c.dest is fluid]
+        plan.list_ok = bool(frho) and [id(x) for x in frho[0].sources] == [
+            id(x) for x in plan.srcs]
+        plan.use_list = plan.use_packed and plan.list_ok
+        plan.list_max_per = 128 if dim == 3 else 64
         if len(plan.srcs) > _abi.SPH_MAX_SRC:
             return None
         return plan
@@ -159,11 +189,59 @@ class WCSPHPlan:
             }
         return self._bufs
 
+    # ------------------------------------------------------ packed copies
+    # FP32 storage: every array carries float4 copies (x,y,z,h), (u,v,w,m)
+    # and (rho,p,cs,p/rho^2) in its current slot order, read by the fast
+    # neighbour passes.  pos4/vel4 are re-packed after every binning of the
+    # array (and from X1 after the stage-2 binning); aux4 is written by the
+    # density pass that precedes every momentum pass.
+    def _packed(self, pa):
+        if not self.use_packed:
+            return None
+        n = pa.get_number_of_particles()
+        pk = self._pk.get(id(pa))
+        if pk is None or pk["n"] != n:
+            mk = lambda: torch.empty((max(n, 1), 4), dtype=torch.float32,  # noqa: E731
+                                     device=pa.device)
+            pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(), "version": None}
+            self._pk[id(pa)] = pk
+        return pk
+
+    def _pack(self, pa, fields=None, force=False):
+        pk = self._packed(pa)
+        if pk is None:
+            return
+        ver = pa.__dict__.get("_bin_version", 0)
+        if not force and pk["version"] == ver:
+            return
+        stream = require_gpu(pk["pos4"])
+        v = pa.view()
+        for f, t in (fields or {}).items():
+            setattr(v, f, ptr(t))
+        _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr())
+        pk["version"] = ver
+
+    def _pack_all(self):
+        for pa in self.particles.arrays:
+            self._pack(pa)
+
+    def _override(self, s1=None):
+        """Per-array field overrides: packed copies and, for the moving
+        array at stage 2, the X1 state."""
+        ov = {}
+        if self.use_packed:
+            for pa in self.particles.arrays:
+                pk = self._pk[id(pa)]
+                ov[id(pa)] = {"pos4": pk["pos4"], "vel4": pk["vel4"], "aux4": pk["aux4"]}
+        if s1 is not None:
+            ov.setdefault(id(self.fluid), {}).update(s1)
+        return ov
+
     def _state0(self):
         pa = self.fluid
         return [pa.get_carray(f) for f in ("x", "y", "z", "u", "v", "w")]
 
-    def _s1_override(self, b):
+    def _s1_fields(self, b):
         """Fields of the stage state X1; components >= dim stay those of X."""
         s0 = self._state0()
         names = ("x", "y", "z", "u", "v", "w")
@@ -171,18 +249,42 @@ class WCSPHPlan:
         for c in range(6):
             comp = c % 3
             ov[names[c]] = b["s1"][c] if comp < self.dim else s0[c]
-        return {id(self.fluid): ov}
+        return ov
+
+    def _nlist(self):
+        """Neighbour list of the moving array, filled by its density pass and
+        read by the momentum pass of the same stage."""
+        if not self.use_list:
+            return None
+        pa = self.fluid
+        n = pa.get_number_of_particles()
+        nl = self._nl
+        if nl is None or nl["n"] != n:
+            maxp = self.list_max_per
+            nl = {"n": n,
+                  "idx": torch.empty(max(n, 1) * maxp, dtype=torch.int32, device=pa.device),
+                  "count": torch.empty(max(n, 1) * _abi.SPH_MAX_SRC, dtype=torch.int32,
+                                       device=pa.device)}
+            st = _abi.SphNlist()
+            st.idx = nl["idx"].data_ptr()
+            st.count = nl["count"].data_ptr()
+            st.max_per = maxp
+            nl["struct"] = st
+            self._nl = nl
+        return nl["struct"]
 
     def _density(self, disp=None, override=None):
         prm = self.prm
         for c in self.rho_calcs:
             pa = c.dest
             rho = pa.get_carray("rho")
+            nl = self._nlist() if pa is self.fluid else None
             if id(pa) in self.tait_arrays:
                 _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
-                             pa.get_carray("cs"), disp=disp, override=override)
+                             pa.get_carray("cs"), disp=disp, override=override, nl=nl)
             else:
-                _ops.density(self.cm, prm, pa, c.sources, rho, disp=disp, override=override)
+                _ops.density(self.cm, prm, pa, c.sources, rho, disp=disp, override=override,
+                             nl=nl)
 
     def _stage(self, stage, dt, b, override=None, disp_in=None, disp_out=None):
         pa = self.fluid
@@ -197,7 +299,7 @@ class WCSPHPlan:
             out = _abi.ptr_array([ptr(t) for t in b["s1"]])
         _abi.call("sph_wcsph_stage", stream, stage, self.cm.grid, self.prm, vd, len(self.srcs),
                   vs, self.flags, _ops.search_radius(self.cm, self.radius_scale), ptr(disp_in),
-                  float(dt), s0, k1, out, ptr(disp_out))
+                  float(dt), s0, k1, out, ptr(disp_out), self._nlist())
 
     # ----------------------------------------------------------------- steps
     def rebin_stage(self, b):
@@ -213,10 +315,14 @@ class WCSPHPlan:
     def rk2_step(self, dt):
         self.cm.update()
         b = self._buffers()
-        self._density()
-        self._stage(1, dt, b)
+        self._pack_all()
+        ov = self._override()
+        self._density(override=ov)
+        self._stage(1, dt, b, override=ov)
         self.rebin_stage(b)
-        ov = self._s1_override(b)
+        s1 = self._s1_fields(b)
+        self._pack(self.fluid, fields=s1, force=True)
+        ov = self._override(s1)
         self._density(override=ov)
         self._stage(2, dt, b, override=ov)
         self.fluid.set_dirty(True)
@@ -225,8 +331,10 @@ class WCSPHPlan:
     def euler_step(self, dt):
         self.cm.update()
         b = self._buffers()
-        self._density()
-        self._stage(0, dt, b)
+        self._pack_all()
+        ov = self._override()
+        self._density(override=ov)
+        self._stage(0, dt, b, override=ov)
         pa = self.fluid
         names = ("x", "y", "z", "u", "v", "w")
         for c in range(6):

@@ -11,11 +11,11 @@ import torch
 from .. import _abi
 from .._device import dtype_code, ptr, require_gpu
 
-# FP32 storage: the pair arithmetic of the neighbour passes is done in FP64
-# by default (SPH_PAIR_F64) so that pressure-gradient sums, which cancel
-# strongly in a fluid at rest, keep a relative error near 1e-7.  Set to
-# SPH_PAIR_NATIVE for all-FP32 pair arithmetic.
-DEFAULT_PAIR_PREC = _abi.SPH_PAIR_F64
+# Pair arithmetic of the neighbour passes for FP32 storage.  All-FP32
+# (SPH_PAIR_NATIVE) keeps the momentum sum of a 3D column at rest within
+# 2e-7 normwise of the FP64 oracle (the contract is 1e-5); SPH_PAIR_F64
+# evaluates each pair in FP64 (2e-8) at about 1.3x the cost.
+DEFAULT_PAIR_PREC = _abi.SPH_PAIR_NATIVE
 
 
 def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, beta=1.0,
@@ -58,14 +58,15 @@ def check_same_dtype(arrays):
         raise TypeError("all arrays of one operation must share a dtype")
 
 
-def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None):
-    """SPHRho over srcs into rho (+ Tait into p, cs)."""
+def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, nl=None):
+    """SPHRho over srcs into rho (+ Tait into p, cs); `nl` (SphNlist)
+    receives the accepted neighbours for a following momentum pass."""
     check_same_dtype([dst] + list(srcs))
     stream = require_gpu(rho)
     vd = views([dst], override)
     vs = views(srcs, override)
     _abi.call("sph_density", stream, cm.grid, prm, vd, len(srcs), vs,
-              search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs))
+              search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs), nl)
 
 
 def tait(prm, dst, p, cs, rho=None):
@@ -76,7 +77,8 @@ def tait(prm, dst, p, cs, rho=None):
     _abi.call("sph_tait", stream, prm, v, ptr(p), ptr(cs))
 
 
-def momentum_xsph(cm, prm, dst, srcs, flags, acc=None, dv=None, disp=None, override=None):
+def momentum_xsph(cm, prm, dst, srcs, flags, acc=None, dv=None, disp=None, override=None,
+                  nl=None):
     """MomentumEquation (sources flagged SPH_SRC_MOMENTUM) and XSPHCorrection
     (flagged SPH_SRC_XSPH) into acc[3] / dv[3]."""
     check_same_dtype([dst] + list(srcs))
@@ -87,7 +89,7 @@ def momentum_xsph(cm, prm, dst, srcs, flags, acc=None, dv=None, disp=None, overr
     a = _abi.ptr_array([ptr(t) for t in acc]) if acc is not None else None
     d = _abi.ptr_array([ptr(t) for t in dv]) if dv is not None else None
     _abi.call("sph_momentum_xsph", stream, cm.grid, prm, vd, len(srcs), vs, fl,
-              search_radius(cm, prm.radius_scale), ptr(disp), a, d)
+              search_radius(cm, prm.radius_scale), ptr(disp), a, d, nl)
 
 
 def update(dtype, n, mode, s, ops, tag=None):

@@ -111,10 +111,14 @@ def test_momentum_xsph_vs_oracle(precision, pair_prec):
     assert rel_norm(d_g, d_o) < tol
 
 
-@pytest.mark.parametrize("precision", ["single", "double"])
-def test_momentum_3d_dam_break_hydrostatic(precision):
-    """3D dam break at rest with hydrostatic-like density: the pressure
-    sum cancels strongly; normwise 1e-5 with FP64 pair arithmetic."""
+@pytest.mark.parametrize("precision,pair_prec", [("single", _abi.SPH_PAIR_F64),
+                                                 ("single", _abi.SPH_PAIR_NATIVE),
+                                                 ("double", _abi.SPH_PAIR_F64)])
+def test_momentum_3d_dam_break_hydrostatic(precision, pair_prec, monkeypatch):
+    """3D dam break at rest: the pressure sum cancels strongly.  Normwise
+    1e-5 with FP64 pair arithmetic; the all-FP32 pair arithmetic is held to
+    1e-4 and its error is printed."""
+    monkeypatch.setattr(_ops, "DEFAULT_PAIR_PREC", pair_prec)
     f, b, info = gen.dam_break_3d(dx=1.0 / 40, column=(12, 12, 24), tank=(30, 12, 24),
                                   seed=2, precision=precision)
     particles = base.Particles([f, b])
@@ -131,7 +135,10 @@ def test_momentum_3d_dam_break_hydrostatic(precision):
     mom.sph("_k1_u40", "_k1_v41", "_k1_w42")
     a_g = np.stack([f.get("_k1_u40"), f.get("_k1_v41"), f.get("_k1_w42")])
     a_o = O.momentum(3, rf, [rf, rb], 0.1, 0.0, 0.1, cell_size=particles.cell_manager.cell_size)
-    assert rel_norm(a_g, a_o) < TOL[precision]
+    err = rel_norm(a_g, a_o)
+    print("hydrostatic momentum %s pair_prec=%d: normwise rel err %.3e, max rel rho err %.3e"
+          % (precision, pair_prec, err, max_rel(f.get("rho"), rho_o)))
+    assert err < (TOL[precision] if pair_prec == _abi.SPH_PAIR_F64 else 1e-4)
 
 
 def test_reference_four_particle_density_and_momentum():

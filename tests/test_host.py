@@ -168,14 +168,14 @@ def test_abi_validates_before_touching_the_gpu():
     for d in range(3):
         g.dims[d] = 1
     rc = L.sph_density(None, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(v), 0, None,
-                       1.0, None, None, None, None)
+                       1.0, None, None, None, None, None)
     assert rc == _abi.SPH_EUNSUPPORTED
     assert b"dimension" in L.sph_last_error()
     prm.dim = 3
     prm.radius_scale = 2.0
     g.cell_size = -1.0
     rc = L.sph_density(None, ctypes.byref(g), ctypes.byref(prm), ctypes.byref(v), 0, None,
-                       1.0, None, None, None, None)
+                       1.0, None, None, None, None, None)
     assert rc == _abi.SPH_EINVAL
     assert L.sph_bin_workspace_bytes(1000, 5000) > 4 * (5000 + 2000)
     rc = L.sph_update(None, 7, 10, 0, 1.0, 0, None, None)
