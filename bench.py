@@ -31,16 +31,19 @@ def parse():
                     help="shrink the lattice (for quick checks); 1.0 = the named workload")
     ap.add_argument("--pair-prec", choices=["f64", "native"], default="native")
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cells-per-radius", type=int, default=1,
+    ap.add_argument("--cells-per-radius", type=int, default=2,
                     help="cell size = 2h / k (neighbour sets are unchanged)")
     ap.add_argument("--no-packed", action="store_true",
                     help="use the SoA kernels instead of the packed fast path")
     ap.add_argument("--no-list", action="store_true",
-                    help="momentum pass searches cells instead of reusing the density list")
+                    help="every pass searches the cells instead of reading neighbour lists")
+    ap.add_argument("--reuse", type=float, default=None, metavar="SKIN",
+                    help="reuse stage-1 neighbour lists in stage 2 with this skin")
     return ap.parse_args()
 
 
-def build_case(scale=1.0, precision="single", cells_per_radius=1, packed=True, nlist=True):
+def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, nlist=True,
+               reuse=None):
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
     from pysph_amd.solver import generators as gen
@@ -63,6 +66,9 @@ def build_case(scale=1.0, precision="single", cells_per_radius=1, packed=True, n
             plan.set_cells_per_radius(cells_per_radius)
         plan.use_packed = plan.use_packed and packed
         plan.use_list = plan.use_list and plan.use_packed and nlist
+        if reuse is not None:
+            plan.list_reuse = True
+            plan.list_skin = reuse
     return s, particles, f, b, info, col, tank
 
 
@@ -119,9 +125,9 @@ def time_kernels(s, nsteps):
         e1.record()
         rec.setdefault(name, []).append((e0, e1))
 
-    def density(disp=None, override=None):
+    def density(disp=None, override=None, **kw):
         st = "2" if disp is not None else "1"
-        timed("density_s" + st, orig_density, disp=disp, override=override)
+        timed("density_s" + st, orig_density, disp=disp, override=override, **kw)
 
     def stage(stage, dt, b, override=None, disp_in=None, disp_out=None):
         timed("stage%d" % stage, orig_stage, stage, dt, b, override=override, disp_in=disp_in,
@@ -188,7 +194,7 @@ def main():
     s, particles, f, b, info, col, tank = build_case(args.scale,
                                                      cells_per_radius=args.cells_per_radius,
                                                      packed=not args.no_packed,
-                                                     nlist=not args.no_list)
+                                                     nlist=not args.no_list, reuse=args.reuse)
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, _ = pair_counts(particles, f, b)
@@ -238,6 +244,7 @@ def main():
                                % args.pair_prec,
                    "cells_per_radius": args.cells_per_radius,
                    "packed": not args.no_packed, "neighbour_list": not args.no_list,
+                   "list_reuse_skin": args.reuse,
                    "fluid_lattice": list(col), "tank_cells": list(tank),
                    "n_fluid": nf, "n_boundary": nb, "dx": info["dx"], "h": info["h"],
                    "dt": info["dt"], "c0": info["co"], "pairs_per_step": pairs_per_step,

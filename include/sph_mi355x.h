@@ -162,18 +162,29 @@ int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
 
 /* ---- neighbour passes -------------------------------------------------- */
 
-/* Optional neighbour list handed from a density pass to the momentum pass
- * of the same stage (same destination, same sources in the same order,
- * unchanged positions).  Entry k of destination i is idx[k*n + i] (n = the
- * destination count), entries grouped by source; count[s*n + i] is the
- * number of accepted neighbours of i in source s.  When a destination has
- * more than max_per neighbours in total, its entries stop at max_per and
- * the momentum pass searches the cells for that destination instead.
- * Only the packed fast path writes and reads the list. */
+/* Optional neighbour list of one destination array, built by a density
+ * pass and read by the later passes of the same step.
+ *   mode 0 (build): sph_density searches the cells and records every source
+ *     within radius*(1+skin) of each destination (a superset of the
+ *     neighbours) while it sums the density over the exact neighbours.
+ *   mode 1 (use):   sph_density, and always sph_momentum_xsph /
+ *     sph_wcsph_stage, visit the recorded sources and apply the exact test.
+ * The list stays exact while no particle has moved by more than
+ * skin*radius/2 since it was built; a pass given disp_dev checks that on
+ * the device and searches the cells instead where it does not hold.
+ * Entry k of destination i is idx[k*n + i] (n = the destination count),
+ * grouped by source in call order; count[s*n + i] holds the number of
+ * entries of source s (idx: max_per*n words, count: SPH_MAX_SRC*n).  A
+ * destination with more than max_per entries keeps max_per of them and is
+ * searched in the cells by the passes that read the list.  The sources and
+ * their order, and the slot order of all arrays, must not change between
+ * the build and the uses.  Only the packed FP32 fast path uses lists. */
 typedef struct sph_nlist {
     int32_t *idx;
     int32_t *count;
     int32_t max_per;
+    int32_t mode;
+    float skin;
     int32_t reserved;
 } sph_nlist;
 

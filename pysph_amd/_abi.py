@@ -52,7 +52,7 @@ class SphParams(ctypes.Structure):
 
 class SphNlist(ctypes.Structure):
     _fields_ = [("idx", _vp), ("count", _vp), ("max_per", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("mode", ctypes.c_int32), ("skin", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
 class SphUpdateOp(ctypes.Structure):

@@ -1,0 +1,107 @@
+// Shared between the SoA neighbour passes (sph_passes.hip) and the packed
+// FP32 fast path (sph_fast.hip): the pass arguments, the exact cut-off test
+// and the Tait equation of state.
+#pragma once
+#include "sph_internal.h"
+
+namespace sph {
+
+// Exact cut-off test in storage precision T, returning the squared distance
+// in CT for the pair arithmetic.
+template <typename T, typename CT>
+struct Cutoff;
+
+template <typename CT>
+struct Cutoff<float, CT> {
+  __device__ __forceinline__ static bool test(float xs, float ys, float zs, float xd, float yd,
+                                              float zd, float r2f, double r2d, CT& d2) {
+    const float dx = xs - xd, dy = ys - yd, dz = zs - zd;
+    const float d2f = dx * dx + dy * dy + dz * dz;
+    if (!Within32::test(xs, ys, zs, xd, yd, zd, d2f, r2f, r2d)) return false;
+    if (sizeof(CT) == 8) {
+      const double ex = (double)xd - (double)xs, ey = (double)yd - (double)ys,
+                   ez = (double)zd - (double)zs;
+      d2 = (CT)(ex * ex + ey * ey + ez * ez);
+    } else {
+      d2 = (CT)d2f;
+    }
+    return true;
+  }
+};
+
+template <typename CT>
+struct Cutoff<double, CT> {
+  __device__ __forceinline__ static bool test(double xs, double ys, double zs, double xd,
+                                              double yd, double zd, float, double r2d,
+                                              CT& d2) {
+#pragma clang fp contract(off)
+    const double dx = xs - xd, dy = ys - yd, dz = zs - zd;
+    const double dd = (dx * dx + dy * dy) + dz * dz;
+    d2 = (CT)dd;
+    return dd < r2d;
+  }
+};
+
+template <typename T>
+struct PassArgs {
+  View<T> dst;
+  View<T> src[SPH_MAX_SRC];
+  unsigned flags[SPH_MAX_SRC];
+  int nsrc;
+  int dim;
+  double rs;   // radius_scale
+  double fac1; // cubic spline fac(1)
+  const float* disp;
+  // density / Tait
+  T* rho;
+  T* p;
+  T* cs;
+  double co, ro, gamma, B;
+  // momentum / XSPH
+  double alpha, beta, eta, eps;
+  T* acc[3];
+  T* dv[3];
+  int do_mom, do_xsph;  // which of the two pair sums the call computes
+  // integrator stage
+  double dt;
+  double g[3];
+  int has_gravity;
+  const T* s0[6];  // x0,y0,z0,u0,v0,w0
+  T* k1[6];        // k1 of momentum (3) and XSPH (3)
+  T* out[6];       // new x,y,z,u,v,w
+  float* disp_out;
+  // neighbour list (superset within radius*(1+nl_skin)) built by a
+  // density scan and read by the later passes of the same step
+  int* nl_idx;
+  int* nl_count;
+  int nl_max;
+  int nl_mode;  // 0 build, 1 use
+  float nl_skin;
+};
+
+// Tait equation of state in FP64 (eos_funcs.pyx:105-116).
+__device__ __forceinline__ void tait(double rho, double co, double ro, double gamma, double B,
+                                     double& p, double& cs) {
+  const double ratio = rho / ro;
+  const double gamma2 = 0.5 * (gamma - 1.0);
+  p = (pow(ratio, gamma) - 1.0) * B;
+  cs = pow(ratio, gamma2) * co;
+}
+
+// True when the destination and every source carry packed FP32 inputs.
+template <typename T>
+static inline bool packed_inputs(const PassArgs<T>& a, bool need_aux) {
+  if (sizeof(T) != 4) return false;
+  if (!a.dst.pos4 || !a.dst.vel4 || (need_aux && !a.dst.aux4)) return false;
+  for (int s = 0; s < a.nsrc; ++s)
+    if (!a.src[s].pos4 || !a.src[s].vel4 || (need_aux && !a.src[s].aux4)) return false;
+  return true;
+}
+
+// Fast-path launchers (sph_fast.hip).  rmax is the largest search radius.
+int32_t fast_density(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,
+                     bool tait, int pair_prec);
+int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,
+                      int mode, int pair_prec);
+
+}  // namespace sph

@@ -13,88 +13,9 @@
 // precision and accepted by the reference's exact FP64 test.
 #include <math.h>
 
-#include "sph_internal.h"
+#include "sph_passes.h"
 
 namespace sph {
-
-// Exact cut-off test in storage precision T, returning the squared distance
-// in CT for the pair arithmetic.
-template <typename T, typename CT>
-struct Cutoff;
-
-template <typename CT>
-struct Cutoff<float, CT> {
-  __device__ __forceinline__ static bool test(float xs, float ys, float zs, float xd, float yd,
-                                              float zd, float r2f, double r2d, CT& d2) {
-    const float dx = xs - xd, dy = ys - yd, dz = zs - zd;
-    const float d2f = dx * dx + dy * dy + dz * dz;
-    if (!Within32::test(xs, ys, zs, xd, yd, zd, d2f, r2f, r2d)) return false;
-    if (sizeof(CT) == 8) {
-      const double ex = (double)xd - (double)xs, ey = (double)yd - (double)ys,
-                   ez = (double)zd - (double)zs;
-      d2 = (CT)(ex * ex + ey * ey + ez * ez);
-    } else {
-      d2 = (CT)d2f;
-    }
-    return true;
-  }
-};
-
-template <typename CT>
-struct Cutoff<double, CT> {
-  __device__ __forceinline__ static bool test(double xs, double ys, double zs, double xd,
-                                              double yd, double zd, float, double r2d,
-                                              CT& d2) {
-#pragma clang fp contract(off)
-    const double dx = xs - xd, dy = ys - yd, dz = zs - zd;
-    const double dd = (dx * dx + dy * dy) + dz * dz;
-    d2 = (CT)dd;
-    return dd < r2d;
-  }
-};
-
-template <typename T>
-struct PassArgs {
-  View<T> dst;
-  View<T> src[SPH_MAX_SRC];
-  unsigned flags[SPH_MAX_SRC];
-  int nsrc;
-  int dim;
-  double rs;   // radius_scale
-  double fac1; // cubic spline fac(1)
-  const float* disp;
-  // density / Tait
-  T* rho;
-  T* p;
-  T* cs;
-  double co, ro, gamma, B;
-  // momentum / XSPH
-  double alpha, beta, eta, eps;
-  T* acc[3];
-  T* dv[3];
-  int do_mom, do_xsph;  // which of the two pair sums the call computes
-  // integrator stage
-  double dt;
-  double g[3];
-  int has_gravity;
-  const T* s0[6];  // x0,y0,z0,u0,v0,w0
-  T* k1[6];        // k1 of momentum (3) and XSPH (3)
-  T* out[6];       // new x,y,z,u,v,w
-  float* disp_out;
-  // neighbour list written by the density pass, read by the momentum pass
-  int* nl_idx;
-  int* nl_count;
-  int nl_max;
-};
-
-// Tait equation of state in FP64 (eos_funcs.pyx:105-116).
-__device__ __forceinline__ void tait(double rho, double co, double ro, double gamma, double B,
-                                     double& p, double& cs) {
-  const double ratio = rho / ro;
-  const double gamma2 = 0.5 * (gamma - 1.0);
-  p = (pow(ratio, gamma) - 1.0) * B;
-  cs = pow(ratio, gamma2) * co;
-}
 
 template <typename T, typename CT, bool TAIT>
 __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) {
@@ -299,358 +220,6 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
   }
 }
 
-// ---------------------------------------------------------------------------
-// Fast path: FP32 storage with packed float4 inputs.
-//
-// One lane per destination, as above, but the pair arithmetic is taken out
-// of the candidate loop.  Candidates are screened from the packed (x,y,z,h)
-// copy (one 16-byte load each); an accepted source index goes into the
-// lane's queue in LDS.  When any lane's queue is full, and at the end of
-// each source array, the wave drains all queues together: every lane runs
-// the pair arithmetic on its own queued sources, so the expensive part runs
-// with nearly all lanes active instead of the ~15% that accept a given
-// candidate.  The wave-uniform control (ballots) keeps every lane in the
-// loops; lanes past the end of the array only idle.
-// ---------------------------------------------------------------------------
-constexpr int kQueue = 32;
-constexpr int kUnroll = 4;
-__device__ constexpr float4 far4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
-
-__device__ __forceinline__ int wave_max(int v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-__device__ __forceinline__ bool accept32(const float4& s, const float4& d, float r2f, double r2d) {
-  const float dx = s.x - d.x, dy = s.y - d.y, dz = s.z - d.z;
-  const float d2f = dx * dx + dy * dy + dz * dz;
-  return Within32::test(s.x, s.y, s.z, d.x, d.y, d.z, d2f, r2f, r2d);
-}
-
-template <typename CT>
-__device__ __forceinline__ CT pair_d2(const float4& d, const float4& s, CT& rx, CT& ry, CT& rz) {
-  rx = (CT)d.x - (CT)s.x;
-  ry = (CT)d.y - (CT)s.y;
-  rz = (CT)d.z - (CT)s.z;
-  return rx * rx + ry * ry + rz * rz;
-}
-
-// Row range of grid row (y, z) between x0 and x1 in one source array; empty
-// when the row lies outside the grid.
-__device__ __forceinline__ void row_range(const Grid& g, const int* __restrict__ cstart, int cx,
-                                          int y, int z, int ring, int& j0, int& j1) {
-  if (y < 0 || y >= g.dims[1] || z < 0 || z >= g.dims[2]) {
-    j0 = j1 = 0;
-    return;
-  }
-  const int x0 = max(cx - ring, 0), x1 = min(cx + ring, g.dims[0] - 1);
-  const int base = (z * g.dims[1] + y) * g.dims[0];
-  j0 = cstart[base + x0];
-  j1 = cstart[base + x1 + 1];
-}
-
-template <typename CT, bool TAIT>
-__global__ __launch_bounds__(kBlock) void density_fast(Grid g, PassArgs<float> a) {
-  __shared__ int qbuf[kBlock / 64][kQueue][64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int* q = &qbuf[wid][0][lane];
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int i = bid * kBlock + threadIdx.x;
-  const bool valid = i < a.dst.n;
-  const bool local = valid && (a.dst.tag == nullptr || a.dst.tag[i] == 0);
-  const float4 pd = valid ? a.dst.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-  const double r = (double)pd.w * a.rs;
-  const double r2d = r * r;
-  const float r2f = (float)r2d;
-  int cx = 0, cy = 0, cz = 0;
-  if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
-  const int ring = ring_of(g, a.disp);
-  const int dim = a.dim;
-  const CT hd = (CT)pd.w;
-  CT rho = CT(0);
-  const int64_t ndst = a.dst.n;
-  const bool write_list = a.nl_idx != nullptr && local;
-  int lpos = 0;  // entries of this destination in the neighbour list
-  for (int s = 0; s < a.nsrc; ++s) {
-    const View<float> S = a.src[s];
-    CT nr = CT(0);
-    int cnt = 0, cnt_s = 0;
-    auto drain = [&]() {
-      const int m = wave_max(cnt);
-      for (int k = 0; k < m; ++k) {
-        if (k < cnt) {
-          const int j = q[k * 64];
-          const float4 ps = S.pos4[j];
-          CT rx, ry, rz;
-          const CT d2 = pair_d2<CT>(pd, ps, rx, ry, rz);
-          const CT hab = CT(0.5) * (hd + (CT)ps.w);
-          const CT inv = CT(1) / hab;
-          const CT qq = sqrt(d2) * inv;
-          nr += (CT)S.vel4[j].w * (w_shape<CT>(qq) * h_dim_inv<CT>(inv, dim));
-          if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
-        }
-      }
-      lpos += cnt;
-      cnt_s += cnt;
-      cnt = 0;
-    };
-    for (int dz = -ring; dz <= ring; ++dz) {
-      for (int dy = -ring; dy <= ring; ++dy) {
-        int j0 = 0, j1 = 0;
-        if (local) row_range(g, S.cstart, cx, cy + dy, cz + dz, ring, j0, j1);
-        // four candidates per iteration: their loads are independent and
-        // in flight together
-        while (__any(j0 < j1)) {
-          if (__any(cnt > kQueue - kUnroll)) drain();
-          float4 c[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u)
-            c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : far4;
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
-            if (j0 + u < j1 && accept32(c[u], pd, r2f, r2d)) {
-              q[cnt * 64] = j0 + u;
-              ++cnt;
-            }
-          }
-          j0 += kUnroll;
-        }
-      }
-    }
-    drain();
-    rho += nr;
-    if (a.nl_count != nullptr && valid) a.nl_count[(int64_t)s * ndst + i] = local ? cnt_s : 0;
-  }
-  if (!valid) return;
-  if (!local) {
-    a.rho[i] = 0.f;
-    if (TAIT) {
-      a.p[i] = 0.f;
-      a.cs[i] = 0.f;
-    }
-    if (a.dst.aux4 != nullptr) a.dst.aux4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-    return;
-  }
-  rho *= (CT)a.fac1;
-  const float rhof = (float)rho;
-  a.rho[i] = rhof;
-  if (TAIT) {
-    double p, cs;
-    tait((double)rhof, a.co, a.ro, a.gamma, a.B, p, cs);
-    a.p[i] = (float)p;
-    a.cs[i] = (float)cs;
-    if (a.dst.aux4 != nullptr) {
-      const float pf = (float)p;
-      a.dst.aux4[i] = make_float4(rhof, pf, (float)cs,
-                                  (float)((double)pf / ((double)rhof * (double)rhof)));
-    }
-  }
-}
-
-template <typename CT, int MODE>
-__global__ __launch_bounds__(kBlock) void momentum_fast(Grid g, PassArgs<float> a) {
-  __shared__ int qbuf[kBlock / 64][kQueue][64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  int* q = &qbuf[wid][0][lane];
-  const int bid = xcd_block(blockIdx.x, gridDim.x);
-  const int i = bid * kBlock + threadIdx.x;
-  const bool valid = i < a.dst.n;
-  const bool local = valid && (a.dst.tag == nullptr || a.dst.tag[i] == 0);
-  const int dim = a.dim;
-  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
-  const float4 pd = valid ? a.dst.pos4[i] : zero4;
-  const float4 vd = valid ? a.dst.vel4[i] : zero4;
-  const float4 ad = valid ? a.dst.aux4[i] : zero4;
-  CT acc[3] = {CT(0), CT(0), CT(0)};
-  CT dv[3] = {CT(0), CT(0), CT(0)};
-  const double r = (double)pd.w * a.rs;
-  const double r2d = r * r;
-  const float r2f = (float)r2d;
-  int cx = 0, cy = 0, cz = 0;
-  if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
-  const int ring = ring_of(g, a.disp);
-  const CT ha = (CT)pd.w, rhoa = (CT)ad.x, ca = (CT)ad.z;
-  const CT Aa = sizeof(CT) == 8 ? (CT)ad.y / (rhoa * rhoa) : (CT)ad.w;
-  const CT alpha = (CT)a.alpha, beta = (CT)a.beta, eta2 = (CT)(a.eta * a.eta);
-  // neighbour list of the density pass: used unless it overflowed
-  const int64_t ndst = a.dst.n;
-  const bool have_list = a.nl_idx != nullptr;
-  bool use_list = false;
-  if (have_list && local) {
-    int total = 0;
-    for (int s = 0; s < a.nsrc; ++s) total += a.nl_count[(int64_t)s * ndst + i];
-    use_list = total <= a.nl_max;
-  }
-  const bool scan = local && !use_list;
-  int lpos = 0;
-  for (int s = 0; s < a.nsrc; ++s) {
-    const View<float> S = a.src[s];
-    const bool do_m = (a.flags[s] & SPH_SRC_MOMENTUM) != 0 && a.do_mom;
-    const bool do_x = (a.flags[s] & SPH_SRC_XSPH) != 0 && a.do_xsph;
-    const int cnt_s = (have_list && local) ? a.nl_count[(int64_t)s * ndst + i] : 0;
-    if (!do_m && !do_x) {
-      lpos += cnt_s;
-      continue;
-    }
-    CT am[3] = {CT(0), CT(0), CT(0)};
-    CT ax[3] = {CT(0), CT(0), CT(0)};
-    auto pair = [&](int j) {
-      const float4 ps = S.pos4[j];
-      const float4 vs = S.vel4[j];
-      const float4 as = S.aux4[j];
-      CT rx, ry, rz;
-      const CT d2 = pair_d2<CT>(pd, ps, rx, ry, rz);
-      const CT hab = CT(0.5) * (ha + (CT)ps.w);
-      const CT inv = CT(1) / hab;
-      const CT rr = sqrt(d2);
-      const CT qq = rr * inv;
-      const CT fac = h_dim_inv<CT>(inv, dim);
-      const CT mb = (CT)vs.w;
-      const CT rhob = (CT)as.x;
-      if (do_m) {
-        const CT vx = (CT)vd.x - (CT)vs.x, vy = (CT)vd.y - (CT)vs.y, vz = (CT)vd.z - (CT)vs.z;
-        const CT dot = vx * rx + vy * ry + vz * rz;
-        const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
-        CT tmp = Aa + Ab;
-        if (dot < CT(0)) {
-          const CT cab = CT(0.5) * (ca + (CT)as.z);
-          const CT rhoab = CT(0.5) * (rhoa + rhob);
-          const CT mu = hab * dot / (d2 + eta2 * hab * hab);
-          tmp += (-alpha * cab * mu + beta * mu * mu) / rhoab;
-        }
-        const CT gf = -mb * tmp * (grad_shape<CT>(qq, rr, hab, inv) * fac);
-        am[0] += gf * rx;
-        am[1] += gf * ry;
-        am[2] += gf * rz;
-      }
-      if (do_x) {
-        const CT rhoab = CT(0.5) * (rhob + rhoa);
-        const CT temp = mb * (w_shape<CT>(qq) * fac) / rhoab;
-        ax[0] += temp * ((CT)vs.x - (CT)vd.x);
-        ax[1] += temp * ((CT)vs.y - (CT)vd.y);
-        ax[2] += temp * ((CT)vs.z - (CT)vd.z);
-      }
-    };
-    if (have_list) {
-      const int m = wave_max(use_list ? cnt_s : 0);
-      for (int k = 0; k < m; ++k) {
-        if (use_list && k < cnt_s) pair(a.nl_idx[(int64_t)(lpos + k) * ndst + i]);
-      }
-      lpos += cnt_s;
-    }
-    if (__any(scan)) {
-      int cnt = 0;
-      auto drain = [&]() {
-        const int m = wave_max(cnt);
-        for (int k = 0; k < m; ++k) {
-          if (k < cnt) pair(q[k * 64]);
-        }
-        cnt = 0;
-      };
-      for (int dz = -ring; dz <= ring; ++dz) {
-        for (int dy = -ring; dy <= ring; ++dy) {
-          int j0 = 0, j1 = 0;
-          if (scan) row_range(g, S.cstart, cx, cy + dy, cz + dz, ring, j0, j1);
-          // four candidates per iteration: their loads are independent and
-          // in flight together
-          while (__any(j0 < j1)) {
-            if (__any(cnt > kQueue - kUnroll)) drain();
-            float4 c[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u)
-              c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : far4;
-#pragma unroll
-            for (int u = 0; u < kUnroll; ++u) {
-              if (j0 + u < j1 && accept32(c[u], pd, r2f, r2d)) {
-                q[cnt * 64] = j0 + u;
-                ++cnt;
-              }
-            }
-            j0 += kUnroll;
-          }
-        }
-      }
-      drain();
-    }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      acc[c] += am[c];
-      dv[c] += ax[c];
-    }
-  }
-  if (!valid) return;
-  {
-    const CT f1 = (CT)a.fac1, fe = (CT)(a.fac1 * a.eps);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      acc[c] *= f1;
-      dv[c] *= fe;
-    }
-  }
-  if (MODE == 0) {
-    for (int c = 0; c < 3; ++c) {
-      if (a.do_mom && a.acc[c] != nullptr) a.acc[c][i] = (float)acc[c];
-      if (a.do_xsph && a.dv[c] != nullptr) a.dv[c][i] = (float)dv[c];
-    }
-    return;
-  }
-  const float dt = (float)a.dt;
-  const float hdt = (float)(0.5 * a.dt);
-  const float vel[3] = {vd.x, vd.y, vd.z};
-  float disp = 0.f;
-  for (int c = 0; c < dim; ++c) {
-    const float x0 = a.s0[c][i];
-    const float u0 = a.s0[3 + c][i];
-    const float km = local ? (float)acc[c] : 0.f;
-    const float kx = local ? (float)dv[c] : 0.f;
-    const float kg = (local && a.has_gravity) ? (float)a.g[c] : 0.f;
-    const float ks = local ? vel[c] : 0.f;
-    float xn, un;
-    if (MODE == 1 || MODE == 3) {
-      un = u0 + km * dt;
-      if (a.has_gravity) un = un + kg * dt;
-      xn = x0 + ks * dt;
-      if (a.do_xsph) xn = xn + kx * dt;
-      if (MODE == 1) {
-        a.k1[c][i] = km;
-        a.k1[3 + c][i] = kx;
-      }
-    } else {
-      const float k1m = a.k1[c][i], k1x = a.k1[3 + c][i];
-      un = u0 + hdt * (k1m + km);
-      if (a.has_gravity) un = un + hdt * (kg + kg);
-      xn = x0 + hdt * (u0 + ks);
-      if (a.do_xsph) xn = xn + hdt * (k1x + kx);
-    }
-    a.out[c][i] = xn;
-    a.out[3 + c][i] = un;
-    disp = fmaxf(disp, fabsf(xn - x0));
-  }
-  if (a.disp_out != nullptr) {
-    for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax((int*)a.disp_out, __float_as_int(disp));
-  }
-}
-
-__global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __restrict__ pos4,
-                                                      float4* __restrict__ vel4) {
-  const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  if (pos4 != nullptr) pos4[i] = make_float4(a.x[i], a.y[i], a.z[i], a.h[i]);
-  if (vel4 != nullptr) vel4[i] = make_float4(a.u[i], a.v[i], a.w[i], a.m[i]);
-}
-
-// True when the destination and every source carry packed FP32 inputs.
-template <typename T>
-static bool packed_inputs(const PassArgs<T>& a, bool need_aux) {
-  if (sizeof(T) != 4) return false;
-  if (!a.dst.pos4 || !a.dst.vel4 || (need_aux && !a.dst.aux4)) return false;
-  for (int s = 0; s < a.nsrc; ++s)
-    if (!a.src[s].pos4 || !a.src[s].vel4 || (need_aux && !a.src[s].aux4)) return false;
-  return true;
-}
-
 template <typename T>
 static int32_t fill_views(PassArgs<T>& a, const sph_view* dst, int32_t nsrc,
                           const sph_view* srcs, const char* who) {
@@ -738,26 +307,6 @@ static void launch_momentum(hipStream_t st, const Grid& G, const PassArgs<T>& a,
   }
 }
 
-template <typename CT>
-static void launch_momentum_fast(hipStream_t st, const Grid& G, const PassArgs<float>& a,
-                                 int mode) {
-  const int nb = nblocks_for(a.dst.n);
-  switch (mode) {
-    case 0:
-      hipLaunchKernelGGL((momentum_fast<CT, 0>), dim3(nb), dim3(kBlock), 0, st, G, a);
-      break;
-    case 1:
-      hipLaunchKernelGGL((momentum_fast<CT, 1>), dim3(nb), dim3(kBlock), 0, st, G, a);
-      break;
-    case 2:
-      hipLaunchKernelGGL((momentum_fast<CT, 2>), dim3(nb), dim3(kBlock), 0, st, G, a);
-      break;
-    default:
-      hipLaunchKernelGGL((momentum_fast<CT, 3>), dim3(nb), dim3(kBlock), 0, st, G, a);
-      break;
-  }
-}
-
 template <typename T>
 static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params* prm,
                             const sph_view* dst, int32_t nsrc, const sph_view* srcs,
@@ -773,26 +322,16 @@ static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params*
   const bool do_tait = p != nullptr && cs != nullptr;
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
-  if (nl != nullptr && packed_inputs(a, false)) {
-    a.nl_idx = nl->idx;
-    a.nl_count = nl->count;
-    a.nl_max = nl->max_per;
-  }
   if (packed_inputs(a, false)) {
-    const int nb = nblocks_for(a.dst.n);
-    const PassArgs<float>& af = reinterpret_cast<const PassArgs<float>&>(a);
-    if (prm->pair_prec == SPH_PAIR_F64) {
-      if (do_tait)
-        hipLaunchKernelGGL((density_fast<double, true>), dim3(nb), dim3(kBlock), 0, st, G, af);
-      else
-        hipLaunchKernelGGL((density_fast<double, false>), dim3(nb), dim3(kBlock), 0, st, G, af);
-    } else {
-      if (do_tait)
-        hipLaunchKernelGGL((density_fast<float, true>), dim3(nb), dim3(kBlock), 0, st, G, af);
-      else
-        hipLaunchKernelGGL((density_fast<float, false>), dim3(nb), dim3(kBlock), 0, st, G, af);
+    if (nl != nullptr) {
+      a.nl_idx = nl->idx;
+      a.nl_count = nl->count;
+      a.nl_max = nl->max_per;
+      a.nl_mode = nl->mode;
+      a.nl_skin = nl->skin;
     }
-    return launch_status("sph_density");
+    return fast_density(st, *g, rmax, reinterpret_cast<const PassArgs<float>&>(a), do_tait,
+                        prm->pair_prec);
   }
   if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
     launch_density<T, double>(st, G, a, do_tait);
@@ -869,13 +408,11 @@ static int32_t momentum_impl(hipStream_t st, int mode, const sph_grid* g, const 
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
       a.nl_max = nl->max_per;
+      a.nl_mode = 1;
+      a.nl_skin = nl->skin;
     }
-    const PassArgs<float>& af = reinterpret_cast<const PassArgs<float>&>(a);
-    if (prm->pair_prec == SPH_PAIR_F64)
-      launch_momentum_fast<double>(st, G, af, mode);
-    else
-      launch_momentum_fast<float>(st, G, af, mode);
-    return launch_status(who);
+    return fast_momentum(st, *g, rmax, reinterpret_cast<const PassArgs<float>&>(a), mode,
+                         prm->pair_prec);
   }
   if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
     launch_momentum<T, double>(st, G, a, mode);
@@ -911,22 +448,6 @@ int32_t sph_density(void* stream, const sph_grid* g, const sph_params* prm, cons
                                rho_dev, p_dev, cs_dev, nlist_out);
   return density_impl<double>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,
                               rho_dev, p_dev, cs_dev, nlist_out);
-}
-
-int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, void* vel4_dev) {
-  SPH_TRY(check_view(a, "sph_pack", vel4_dev != nullptr, false, false));
-  if (a->dtype != SPH_F32) {
-    set_error("sph_pack: packed copies exist for FP32 storage only");
-    return SPH_EUNSUPPORTED;
-  }
-  if (vel4_dev != nullptr && a->n > 0 && !a->m) {
-    set_error("sph_pack: m is required for vel4");
-    return SPH_EINVAL;
-  }
-  if (a->n == 0) return SPH_OK;
-  hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, as_stream(stream),
-                     to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev);
-  return launch_status("sph_pack");
 }
 
 int32_t sph_tait(void* stream, const sph_params* prm, const sph_view* a, void* p_dev,

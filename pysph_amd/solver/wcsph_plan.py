@@ -16,8 +16,13 @@ Per RK2 step the plan issues, on one stream:
     re-bin F (counting sort + permutation)          sph_bin / sph_permute
     per array: density + Tait at X                  sph_density
     momentum + XSPH + predictor update -> X1, k1    sph_wcsph_stage(1)
-    re-bin F by X1, permuting X, X1, k1 along       sph_bin / sph_permute
     per array: density + Tait at X1                 sph_density
+With FP32 storage the density pass of each stage also records every
+array's neighbours, and the momentum pass of the stage reads them instead
+of searching the cells again.  Optionally (list_reuse) the stage-1 lists,
+built with a skin, serve stage 2 too while the predictor's displacement
+stays within the skin (checked on the device), and stage 2 keeps the
+binning.
     momentum + XSPH + final update -> X (in place)  sph_wcsph_stage(2)
 
 Stage 2 re-bins the moving array by the predicted positions X1 and carries
@@ -76,10 +81,15 @@ class WCSPHPlan:
         self._bufs = None
         self.steps = 0
         self._pk = {}
-        self._nl = None
+        self._nl = {}
         self.use_list = False
         self.list_ok = False
         self.list_max_per = 128
+        # list_reuse: the stage-1 lists (radius (1 + skin) * search radius)
+        # serve stage 2 too, valid while particles move less than
+        # skin * radius / 2; otherwise every stage builds exact lists
+        self.list_reuse = False
+        self.list_skin = 0.0
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
 
@@ -168,7 +178,7 @@ class WCSPHPlan:
         # momentum pass when both see the same sources in the same order
         frho = [c for c in rho_calcs if c.dest is fluid]
         plan.list_ok = bool(frho) and [id(x) for x in frho[0].sources] == [
-            id(x) for x in plan.srcs]
+            id(x) for x in plan.srcs] and all(len(c.sources) > 0 for c in rho_calcs)
         plan.use_list = plan.use_packed and plan.list_ok
         plan.list_max_per = 128 if dim == 3 else 64
         if len(plan.srcs) > _abi.SPH_MAX_SRC:
@@ -251,34 +261,34 @@ class WCSPHPlan:
             ov[names[c]] = b["s1"][c] if comp < self.dim else s0[c]
         return ov
 
-    def _nlist(self):
-        """Neighbour list of the moving array, filled by its density pass and
-        read by the momentum pass of the same stage."""
+    def _nlist(self, pa, mode):
+        """Neighbour list of array `pa` (superset radius (1+skin) * radius):
+        mode 0 = built by the density pass, 1 = read by the later passes."""
         if not self.use_list:
             return None
-        pa = self.fluid
         n = pa.get_number_of_particles()
-        nl = self._nl
+        nl = self._nl.get(id(pa))
         if nl is None or nl["n"] != n:
             maxp = self.list_max_per
             nl = {"n": n,
                   "idx": torch.empty(max(n, 1) * maxp, dtype=torch.int32, device=pa.device),
                   "count": torch.empty(max(n, 1) * _abi.SPH_MAX_SRC, dtype=torch.int32,
                                        device=pa.device)}
-            st = _abi.SphNlist()
-            st.idx = nl["idx"].data_ptr()
-            st.count = nl["count"].data_ptr()
-            st.max_per = maxp
-            nl["struct"] = st
-            self._nl = nl
-        return nl["struct"]
+            self._nl[id(pa)] = nl
+        st = _abi.SphNlist()
+        st.idx = nl["idx"].data_ptr()
+        st.count = nl["count"].data_ptr()
+        st.max_per = self.list_max_per
+        st.mode = mode
+        st.skin = self.list_skin
+        return st
 
-    def _density(self, disp=None, override=None):
+    def _density(self, disp=None, override=None, list_mode=0):
         prm = self.prm
         for c in self.rho_calcs:
             pa = c.dest
             rho = pa.get_carray("rho")
-            nl = self._nlist() if pa is self.fluid else None
+            nl = self._nlist(pa, list_mode)
             if id(pa) in self.tait_arrays:
                 _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
                              pa.get_carray("cs"), disp=disp, override=override, nl=nl)
@@ -299,7 +309,7 @@ class WCSPHPlan:
             out = _abi.ptr_array([ptr(t) for t in b["s1"]])
         _abi.call("sph_wcsph_stage", stream, stage, self.cm.grid, self.prm, vd, len(self.srcs),
                   vs, self.flags, _ops.search_radius(self.cm, self.radius_scale), ptr(disp_in),
-                  float(dt), s0, k1, out, ptr(disp_out), self._nlist())
+                  float(dt), s0, k1, out, ptr(disp_out), self._nlist(pa, 1))
 
     # ----------------------------------------------------------------- steps
     def rebin_stage(self, b):
@@ -317,14 +327,31 @@ class WCSPHPlan:
         b = self._buffers()
         self._pack_all()
         ov = self._override()
-        self._density(override=ov)
-        self._stage(1, dt, b, override=ov)
-        self.rebin_stage(b)
-        s1 = self._s1_fields(b)
-        self._pack(self.fluid, fields=s1, force=True)
-        ov = self._override(s1)
-        self._density(override=ov)
-        self._stage(2, dt, b, override=ov)
+        if self.use_list and self.list_reuse:
+            # stage 1 builds the neighbour lists (superset radius) that the
+            # stage-1 momentum pass and every stage-2 pass read; stage 2 keeps
+            # the binning and checks the predictor's displacement on the
+            # device against the list skin
+            stream = require_gpu(b["disp"])
+            _abi.call("sph_fill_i32", stream, b["disp"].data_ptr(), 1, 0)
+            self._density(override=ov, list_mode=0)
+            self._stage(1, dt, b, override=ov, disp_out=b["disp"])
+            s1 = self._s1_fields(b)
+            self._pack(self.fluid, fields=s1, force=True)
+            ov = self._override(s1)
+            self._density(disp=b["disp"], override=ov, list_mode=1)
+            self._stage(2, dt, b, override=ov, disp_in=b["disp"])
+        else:
+            # each stage bins its own positions; with lists, the density pass
+            # of each stage records the neighbours its momentum pass reads
+            self._density(override=ov, list_mode=0)
+            self._stage(1, dt, b, override=ov)
+            self.rebin_stage(b)
+            s1 = self._s1_fields(b)
+            self._pack(self.fluid, fields=s1, force=True)
+            ov = self._override(s1)
+            self._density(override=ov, list_mode=0)
+            self._stage(2, dt, b, override=ov)
         self.fluid.set_dirty(True)
         self.steps += 1
 
@@ -333,7 +360,7 @@ class WCSPHPlan:
         b = self._buffers()
         self._pack_all()
         ov = self._override()
-        self._density(override=ov)
+        self._density(override=ov, list_mode=0)
         self._stage(0, dt, b, override=ov)
         pa = self.fluid
         names = ("x", "y", "z", "u", "v", "w")

@@ -179,7 +179,7 @@ def test_conservation_mass_and_momentum():
 
 
 def _dam3d_solver(precision="single", packed=True, cells_per_radius=1, seed=4, nlist=True,
-                  max_per=None):
+                  max_per=None, skin=None):
     f, b, info = gen.dam_break_3d(dx=1.0 / 40, column=(10, 10, 20), tank=(24, 10, 20),
                                   seed=seed, precision=precision)
     particles = base.Particles([f, b])
@@ -194,19 +194,25 @@ def _dam3d_solver(precision="single", packed=True, cells_per_radius=1, seed=4, n
     plan.use_list = plan.use_list and plan.use_packed and nlist
     if max_per is not None:
         plan.list_max_per = max_per
+    if skin is not None:
+        plan.list_reuse = True
+        plan.list_skin = skin
     s.set_time_step(info["dt"])
     return s, f, b, info, particles
 
 
-@pytest.mark.parametrize("cells_per_radius,nlist,max_per", [(1, True, None), (2, True, None),
-                                                            (3, False, None), (2, True, 20)])
-def test_fast_path_matches_soa_path_3d(cells_per_radius, nlist, max_per):
-    """The packed fast kernels (with or without the density pass's
-    neighbour list, and with a list too short for most particles, which
-    falls back to the cell search) and the SoA kernels compute the same
-    sums: 5 RK2 steps of a 3D dam break agree to FP32 rounding."""
+@pytest.mark.parametrize("cells_per_radius,nlist,max_per,skin", [
+    (1, True, None, None), (2, True, None, None), (3, False, None, None),
+    (2, True, 20, None),    # lists too short: most particles search the cells
+    (2, True, None, 0.0),   # reused lists without skin: stage 2 searches the cells
+    (1, True, None, 0.3),   # reused lists with a skin
+    (2, True, 20, 0.3)])
+def test_fast_path_matches_soa_path_3d(cells_per_radius, nlist, max_per, skin):
+    """The packed fast kernels (cell scans, superset neighbour lists, and
+    both fallbacks of the lists) and the SoA kernels compute the same sums:
+    5 RK2 steps of a 3D dam break agree to FP32 rounding."""
     s1, f1, _, info, _ = _dam3d_solver(packed=True, cells_per_radius=cells_per_radius,
-                                       nlist=nlist, max_per=max_per)
+                                       nlist=nlist, max_per=max_per, skin=skin)
     s2, f2, _, _, _ = _dam3d_solver(packed=False)
     assert s1.integrator.plan.use_packed and not s2.integrator.plan.use_packed
     assert s1.integrator.plan.use_list == nlist
