@@ -35,14 +35,17 @@ def parse():
                     help="cell size = 2h / k (neighbour sets are unchanged)")
     ap.add_argument("--no-packed", action="store_true",
                     help="use the SoA kernels instead of the packed fast path")
-    ap.add_argument("--no-list", action="store_true",
-                    help="every pass searches the cells instead of reading neighbour lists")
+    ap.add_argument("--list", action="store_true",
+                    help="the density pass records neighbour lists for the momentum pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device-to-device 3 times (calibrates the "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--reuse", type=float, default=None, metavar="SKIN",
                     help="reuse stage-1 neighbour lists in stage 2 with this skin")
     return ap.parse_args()
 
 
-def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, nlist=True,
+def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, nlist=False,
                reuse=None):
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
@@ -65,10 +68,8 @@ def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, n
         if cells_per_radius != 1:
             plan.set_cells_per_radius(cells_per_radius)
         plan.use_packed = plan.use_packed and packed
-        plan.use_list = plan.use_list and plan.use_packed and nlist
-        if reuse is not None:
-            plan.list_reuse = True
-            plan.list_skin = reuse
+        if nlist or reuse is not None:
+            plan.enable_lists(True, skin=reuse, reuse=reuse is not None)
     return s, particles, f, b, info, col, tank
 
 
@@ -89,8 +90,27 @@ def pair_counts(particles, f, b):
             counts[(dst.name, src.name)] = int(c.sum().item())
     p_rho = sum(counts.values())
     p_m = counts[("fluid", "fluid")] + counts[("fluid", "boundary")]
-    p_x = counts[("fluid", "fluid")]
-    return p_rho, p_m, p_x
+    # density pairs of the fluid destinations (= p_m: same source arrays)
+    return p_rho, p_m, p_m
+
+
+def config_key(args):
+    """Kernel configuration the traffic figures belong to."""
+    return "cpr%d-%s-%s%s" % (args.cells_per_radius, "packed" if not args.no_packed else "soa",
+                              "list" if args.list else "nolist",
+                              "" if args.reuse is None else "-reuse%g" % args.reuse)
+
+
+def kernel_names(plan):
+    """Kernel symbols behind each pass (as rocprofv3 lists them)."""
+    if not plan.use_packed:
+        return {"density_fluid": "sph::density_kernel<float, ...>",
+                "stage1": "sph::momentum_kernel<float, 1>",
+                "stage2": "sph::momentum_kernel<float, 2>"}
+    build = "true" if plan.use_list else "false"
+    mom = "momentum_list" if plan.use_list else "momentum_scan"
+    return {"density_fluid": "sph::density_scan<float, true, %s>" % build,
+            "stage1": "sph::%s<float, 1>" % mom, "stage2": "sph::%s<float, 2>" % mom}
 
 
 def kernel_bytes(nf, nb):
@@ -109,38 +129,129 @@ def kernel_bytes(nf, nb):
     }
 
 
-def time_kernels(s, nsteps):
-    """Per-launch durations (ms) of the plan's kernels over nsteps steps,
-    with events recorded on the launch stream around each launch."""
+class LaunchTimer:
+    """Stream events recorded immediately around the density and stage
+    entry points (on torch's current stream, the launch stream), keyed by
+    pass and destination array."""
+
+    def __init__(self, nf):
+        import torch
+        self.torch = torch
+        self.nf = nf
+        self.rec = {}
+        self.cur = None
+
+    def key(self, name, args):
+        if name == "sph_density":
+            return "density_fluid" if args[3][0].n == self.nf else "density_boundary"
+        if name == "sph_wcsph_stage":
+            return "stage%d" % args[1]
+        return None
+
+    def before(self, name, args):
+        k = self.key(name, args)
+        if k is not None:
+            e0 = self.torch.cuda.Event(enable_timing=True)
+            e0.record()
+            self.cur = (k, e0)
+
+    def after(self, name, args):
+        if self.cur is not None:
+            e1 = self.torch.cuda.Event(enable_timing=True)
+            e1.record()
+            self.rec.setdefault(self.cur[0], []).append((self.cur[1], e1))
+            self.cur = None
+
+    def durations(self):
+        return {k: [a.elapsed_time(b) for a, b in v] for k, v in self.rec.items()}
+
+
+def time_kernels(s, nf, nsteps):
+    """Launch durations (ms) of the plan's passes over nsteps steps."""
     import torch
-    plan = s.integrator.plan
-    rec = {}
-    orig_density, orig_stage = plan._density, plan._stage
-
-    def timed(name, fn, *a, **k):
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        e0.record()
-        fn(*a, **k)
-        e1.record()
-        rec.setdefault(name, []).append((e0, e1))
-
-    def density(disp=None, override=None, **kw):
-        st = "2" if disp is not None else "1"
-        timed("density_s" + st, orig_density, disp=disp, override=override, **kw)
-
-    def stage(stage, dt, b, override=None, disp_in=None, disp_out=None):
-        timed("stage%d" % stage, orig_stage, stage, dt, b, override=override, disp_in=disp_in,
-              disp_out=disp_out)
-
-    plan._density, plan._stage = density, stage
+    from pysph_amd import _abi
+    timer = LaunchTimer(nf)
+    # a few untimed steps first, without synchronising, so that the launch
+    # queue is ahead of the GPU and no event waits on the host
+    for _ in range(3):
+        s.integrator.integrate(s.dt)
+    _abi.tracer = timer
     try:
         for _ in range(nsteps):
             s.integrator.integrate(s.dt)
         torch.cuda.synchronize()
     finally:
-        plan._density, plan._stage = orig_density, orig_stage
-    return {k: float(np.mean([a.elapsed_time(b) for a, b in v])) for k, v in rec.items()}
+        _abi.tracer = None
+    return timer.durations()
+
+
+def batch_launch_ms(s, dom, reps=20):
+    """Average launch duration of the dominant pass's kernel: `reps`
+    back-to-back launches between two events on the launch stream, after
+    one full step.  The launches are idempotent: the stage-1 kernel writes
+    X1 and k1 from X, the density pass rewrites rho, p, cs from positions."""
+    import torch
+    from pysph_amd.sph import _ops
+    plan = s.integrator.plan
+    s.integrator.integrate(s.dt)
+    b = plan._buffers()
+    ov = plan._override()
+    if dom == "momentum_stage":
+        key = "stage1"
+
+        def launch():
+            plan._stage(1, s.dt, b, override=ov)
+    else:
+        key = "density_fluid"
+        c = next(c for c in plan.rho_calcs if c.dest is plan.fluid)
+        pa = c.dest
+
+        def launch():
+            _ops.density(plan.cm, plan.prm, pa, c.sources, pa.get_carray("rho"),
+                         pa.get_carray("p"), pa.get_carray("cs"), override=ov)
+    launch()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        launch()
+    e1.record()
+    torch.cuda.synchronize()
+    return key, e0.elapsed_time(e1) / reps
+
+
+def source_digest():
+    """SHA-256 of the kernel sources (keys the committed traffic figures)."""
+    import hashlib
+    d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pysph_amd", "csrc")
+    h = hashlib.sha256()
+    for fn in sorted(os.listdir(d)):
+        if fn.endswith((".hip", ".h")):
+            with open(os.path.join(d, fn), "rb") as fh:
+                h.update(fn.encode() + b"\0" + fh.read())
+    return h.hexdigest()
+
+
+def measured_traffic(config_key):
+    """HBM bytes per launch from the rocprofv3 FETCH_SIZE / WRITE_SIZE
+    summary under profiles/, when it was taken on the current sources."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "dam3d_hbm_traffic.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("traffic-json:"):
+                    rec = json.loads(line[len("traffic-json:"):])
+                    if rec.get("source_sha256") != source_digest():
+                        return None, "stale (kernel sources changed since the counter runs)"
+                    t = rec.get("configs", {}).get(config_key)
+                    if t is None:
+                        return None, "no counter run for this configuration"
+                    return t, path
+    except OSError:
+        pass
+    return None, "no counter summary"
 
 
 def cpu_baseline(info):
@@ -168,17 +279,24 @@ def cpu_baseline(info):
     off, _ = O.nnps_csr(rf, None, cell_size=cs)
     off2, _ = O.nnps_csr(rf, rb, cell_size=cs)
     p_m = int(off[-1] + off2[-1])
-    O.set_threads(1)
-    t0 = time.perf_counter()
-    O.scheme_step(sch, [rf, rb], 1)
-    t = time.perf_counter() - t0
     pairs_per_step = 2 * (npairs + p_m)
-    return {"value": pairs_per_step / t, "unit": "pair-interactions/s", "cores": 1,
+    out = {}
+    for threads, steps in ((1, 6), (len(os.sched_getaffinity(0)), 6)):
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        O.scheme_step(sch, [rf, rb], steps)
+        t = (time.perf_counter() - t0) / steps
+        out[threads] = {
+            "value": pairs_per_step / t, "unit": "pair-interactions/s", "cores": threads,
             "kind": "port",
-            "sample": "one RK2 step of a 40x40x81-lattice dam break (%d fluid + %d boundary "
-                      "particles, same dx/h/c0/dt) with the FP64 oracle, single thread, "
-                      "%.1f s" % (len(rf["x"]), len(rb["x"]), t),
+            "sample": "%d RK2 steps of a 40x40x81-lattice dam break (%d fluid + %d boundary "
+                      "particles, same dx/h/c0/dt) with the FP64 oracle, %d thread(s), "
+                      "%.2f s per step" % (steps, len(rf["x"]), len(rb["x"]), threads, t),
             "seconds_per_step": t}
+    O.set_threads(1)
+    base = out[1]
+    base["all_cores"] = out[max(out)]
+    return base
 
 
 def main():
@@ -188,16 +306,24 @@ def main():
               "rank's share); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         sys.exit(2)
     import torch
+    if args.copy_probe:
+        a = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
+        return
     from pysph_amd.sph import _ops
     from pysph_amd import _abi
     _ops.DEFAULT_PAIR_PREC = _abi.SPH_PAIR_F64 if args.pair_prec == "f64" else _abi.SPH_PAIR_NATIVE
     s, particles, f, b, info, col, tank = build_case(args.scale,
                                                      cells_per_radius=args.cells_per_radius,
                                                      packed=not args.no_packed,
-                                                     nlist=not args.no_list, reuse=args.reuse)
+                                                     nlist=args.list, reuse=args.reuse)
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
-    p_rho0, p_m0, _ = pair_counts(particles, f, b)
+    p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
     for _ in range(args.warmup):
         s.integrator.integrate(s.dt)
     torch.cuda.synchronize()
@@ -213,18 +339,30 @@ def main():
     gpu_ms = e0.elapsed_time(e1)
     t = wall
     particles.cell_manager.verify_jumps()
-    p_rho1, p_m1, _ = pair_counts(particles, f, b)
+    p_rho1, p_m1, p_rho_fluid1 = pair_counts(particles, f, b)
     p_rho, p_m = 0.5 * (p_rho0 + p_rho1), 0.5 * (p_m0 + p_m1)
     pairs_per_step = 2 * (p_rho + p_m)
     value = pairs_per_step * args.steps / t
-    kt = time_kernels(s, args.profile_steps)
+    kt = time_kernels(s, nf, args.profile_steps)
     kb = kernel_bytes(nf, nb)
-    dur = {"density_fluid": None, "stage1": kt.get("stage1"), "stage2": kt.get("stage2")}
-    # dominant kernel: the longer of the two stage kernels
-    dom = max(("stage1", "stage2"), key=lambda k: kt.get(k, 0.0))
-    dom_ms = kt[dom]
-    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
-    flops_dom = 80.0 * p_m + 0.0
+    plan = s.integrator.plan
+    names = kernel_names(plan)
+    # passes grouped by kernel: the density kernel on fluid destinations
+    # (twice per step) and the momentum/stage kernel (stage 1 + stage 2)
+    fam = {"density_fluid": ["density_fluid"], "momentum_stage": ["stage1", "stage2"]}
+    per_step = {f: sum(sum(kt.get(k, [])) for k in ks) / args.profile_steps
+                for f, ks in fam.items()}
+    dom = max(per_step, key=per_step.get)
+    key, dom_ms = batch_launch_ms(s, dom, reps=20)
+    dom_bytes = float(kb[key])
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    cfg_key = config_key(args)
+    traffic, traffic_src = measured_traffic(cfg_key)
+    dom_traffic = traffic.get(key) if traffic else None
+    # algorithmic flops of the dominant launch (20 per density pair, 80 per
+    # momentum pair)
+    p_rho_f = 0.5 * (p_rho_fluid0 + p_rho_fluid1)
+    flops_dom = 20.0 * p_rho_f if dom == "density_fluid" else 80.0 * p_m
     out = {
         "metric": "pair-interactions/s",
         "value": value,
@@ -243,21 +381,26 @@ def main():
         "config": {"workload": "3D dam break, RK2, FP32 storage (pair arithmetic %s)"
                                % args.pair_prec,
                    "cells_per_radius": args.cells_per_radius,
-                   "packed": not args.no_packed, "neighbour_list": not args.no_list,
+                   "packed": not args.no_packed, "neighbour_list": args.list,
                    "list_reuse_skin": args.reuse,
                    "fluid_lattice": list(col), "tank_cells": list(tank),
                    "n_fluid": nf, "n_boundary": nb, "dx": info["dx"], "h": info["h"],
                    "dt": info["dt"], "c0": info["co"], "pairs_per_step": pairs_per_step,
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "momentum_kernel (%s)" % dom,
+        "roofline": {"bound": "hbm", "kernel": names[key], "pass": key,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": kb[dom], "avg_launch_ms": dom_ms},
-        "valu": {"kernel": dom, "flops_per_launch": flops_dom,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": dom_traffic,
+                     "traffic_source": traffic_src if traffic else traffic_src,
+                     "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms,
+                     "launches_timed": 20},
+        "valu": {"kernel": key, "flops_per_launch": flops_dom,
                  "achieved_tflops": flops_dom / (dom_ms * 1e-3) / 1e12,
                  "peak_tflops": FP32_PEAK_TFLOPS,
                  "frac": flops_dom / (dom_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
-        "kernel_ms": kt,
+        # per-launch events inside the step loop (each event pair adds its
+        # own cost to the launch it brackets)
+        "kernel_ms_events_in_loop": {k: float(np.mean(v)) for k, v in kt.items()},
+        "pass_ms_per_step": per_step,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(info)

@@ -8,7 +8,8 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libsph_mi355x.so")
+# SPH_MI355X_LIB selects another build of the library (e.g. a tuning variant)
+LIB_PATH = os.environ.get("SPH_MI355X_LIB") or os.path.join(HERE, "libsph_mi355x.so")
 
 SPH_OK = 0
 SPH_EINVAL = -1
@@ -136,10 +137,23 @@ def load(path=LIB_PATH):
     return L
 
 
+# Optional launch tracer: an object with before(name, args) and
+# after(name, args), called around every entry-point call (used to bracket
+# launches with stream events when timing kernels).
+tracer = None
+
+
 def call(name, *args):
     """Call an ABI entry point and raise SphError on a non-zero return."""
     L = load()
-    rc = getattr(L, name)(*args)
+    fn = getattr(L, name)
+    t = tracer
+    if t is None:
+        rc = fn(*args)
+    else:
+        t.before(name, args)
+        rc = fn(*args)
+        t.after(name, args)
     if rc != SPH_OK:
         msg = L.sph_last_error().decode(errors="replace")
         cls = JumpToleranceError if rc == SPH_EJUMP else SphError

@@ -27,21 +27,26 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
+def build(verbose=False, out=LIB, defines=()):
+    """Compile every source and link `out`; `defines` are extra -D macros
+    (tuning constants) for variant builds."""
     objs = []
-    os.makedirs(os.path.join(HERE, "_obj"), exist_ok=True)
+    objdir = os.path.join(HERE, "_obj", os.path.basename(out).replace(".so", ""))
+    os.makedirs(objdir, exist_ok=True)
+    extra = ["-D" + d for d in defines]
     for src in SOURCES:
-        obj = os.path.join(HERE, "_obj", src.replace(".hip", ".o"))
-        cmd = [HIPCC] + [f for f in FLAGS if f != "-shared"] + ["-c", os.path.join(CSRC, src), "-o", obj]
+        obj = os.path.join(objdir, src.replace(".hip", ".o"))
+        cmd = ([HIPCC] + [f for f in FLAGS if f != "-shared"] + extra +
+               ["-c", os.path.join(CSRC, src), "-o", obj])
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.check_call(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 if __name__ == "__main__":

@@ -28,8 +28,15 @@
 
 namespace sph {
 
-constexpr int kQueue = 32;
-constexpr int kUnroll = 4;
+// LDS queue depth per lane and candidate loads in flight per lane
+#ifndef SPH_QUEUE
+#define SPH_QUEUE 20
+#endif
+#ifndef SPH_UNROLL
+#define SPH_UNROLL 4
+#endif
+constexpr int kQueue = SPH_QUEUE;
+constexpr int kUnroll = SPH_UNROLL;
 __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
 __device__ __forceinline__ int wave_max(int v) {
@@ -213,6 +220,7 @@ __global__ __launch_bounds__(kBlock) void density_scan(Grid g, PassArgs<float> a
   const bool exact_list = !BUILD || a.nl_skin == 0.f;
   const double rl = BUILD ? r * (1.0 + (double)a.nl_skin) : r;
   const float rl2f = (float)(rl * rl) * 1.00001f;
+  const bool write_list = BUILD && local;
   int cx = 0, cy = 0, cz = 0;
   if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
   const int ring = ring_of(g, a.disp);
@@ -221,7 +229,6 @@ __global__ __launch_bounds__(kBlock) void density_scan(Grid g, PassArgs<float> a
   const int dim = a.dim;
   const CT hd = (CT)pd.w;
   const int64_t ndst = a.dst.n;
-  const bool write_list = BUILD && local;
   CT rho = CT(0);
   int lpos = 0;
   for (int s = 0; s < a.nsrc; ++s) {

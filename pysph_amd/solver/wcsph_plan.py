@@ -93,6 +93,18 @@ class WCSPHPlan:
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
 
+    def enable_lists(self, on=True, skin=None, reuse=None):
+        """Density passes record neighbour lists that the momentum pass of
+        the same stage reads (when the density and momentum sources are the
+        same arrays); with reuse, stage-1 lists of radius (1 + skin) serve
+        stage 2 as well."""
+        self.use_list = bool(on) and self.use_packed and self.list_ok
+        if skin is not None:
+            self.list_skin = float(skin)
+        if reuse is not None:
+            self.list_reuse = bool(reuse)
+        return self.use_list
+
     def set_cells_per_radius(self, k):
         """Cell size = search radius / k (k = 1 is the reference's 2 max h).
         Smaller cells test fewer candidates per neighbour at the price of
@@ -179,7 +191,10 @@ class WCSPHPlan:
         frho = [c for c in rho_calcs if c.dest is fluid]
         plan.list_ok = bool(frho) and [id(x) for x in frho[0].sources] == [
             id(x) for x in plan.srcs] and all(len(c.sources) > 0 for c in rho_calcs)
-        plan.use_list = plan.use_packed and plan.list_ok
+        # lists are opt-in (enable_lists): on MI355X the second cell search
+        # of the momentum pass costs about what writing and reading the
+        # list costs, and searching moves no list through HBM
+        plan.use_list = False
         plan.list_max_per = 128 if dim == 3 else 64
         if len(plan.srcs) > _abi.SPH_MAX_SRC:
             return None
@@ -288,7 +303,8 @@ class WCSPHPlan:
         for c in self.rho_calcs:
             pa = c.dest
             rho = pa.get_carray("rho")
-            nl = self._nlist(pa, list_mode)
+            # only the moving array has a momentum pass that reads a list
+            nl = self._nlist(pa, list_mode) if pa is self.fluid else None
             if id(pa) in self.tait_arrays:
                 _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
                              pa.get_carray("cs"), disp=disp, override=override, nl=nl)

@@ -191,7 +191,7 @@ def _dam3d_solver(precision="single", packed=True, cells_per_radius=1, seed=4, n
     if cells_per_radius != 1:
         plan.set_cells_per_radius(cells_per_radius)
     plan.use_packed = plan.use_packed and packed
-    plan.use_list = plan.use_list and plan.use_packed and nlist
+    plan.enable_lists(nlist)
     if max_per is not None:
         plan.list_max_per = max_per
     if skin is not None:
