@@ -76,6 +76,7 @@ Grid make_grid(const sph_grid& g, double rmax) {
   Grid G;
   G.cs = g.cell_size;
   G.cs_f = (float)g.cell_size;
+  G.inv_cs_f = (float)(1.0 / g.cell_size);
   for (int d = 0; d < 3; ++d) {
     G.lo[d] = g.lo[d];
     G.dims[d] = g.dims[d];

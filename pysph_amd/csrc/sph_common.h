@@ -30,6 +30,7 @@ struct Grid {
   int lo[3];     // integer cell coordinate of grid cell (0,0,0)
   int dims[3];
   float cs_f;
+  float inv_cs_f;  // 1 / cs_f (row culling only; conservative by its margin)
   double rmax;   // largest search radius of the call (radius_scale * max h)
   int ring0;     // ceil(rmax / cs)
 };

@@ -128,7 +128,7 @@ __device__ __forceinline__ Cull make_cull(const float4& pd, double rc, const Gri
   c.yd = pd.y;
   c.zd = pd.z;
   const float scale = fabsf(pd.x) + fabsf(pd.y) + fabsf(pd.z) + g.cs_f;
-  const float r = (float)rc * 1.00001f + 4e-7f * scale;
+  const float r = (float)rc * 1.00001f + 8e-7f * scale;
   c.rc2 = r * r;
   return c;
 }
@@ -145,8 +145,8 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   const float w2 = cu.rc2 - gy * gy - gz * gz;
   if (!(w2 >= 0.f)) return;
   const float w = sqrtf(w2);
-  const float fa = floorf((cu.xd - w) / g.cs_f) - (float)g.lo[0];
-  const float fb = floorf((cu.xd + w) / g.cs_f) - (float)g.lo[0];
+  const float fa = floorf((cu.xd - w) * g.inv_cs_f) - (float)g.lo[0];
+  const float fb = floorf((cu.xd + w) * g.inv_cs_f) - (float)g.lo[0];
   int x0 = max(cx - ring, 0), x1 = min(cx + ring, g.dims[0] - 1);
   if (fa > (float)x1 || fb < (float)x0) return;  // NaN positions keep the ring
   if (fa > (float)x0) x0 = (int)fa;
