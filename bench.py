@@ -24,6 +24,11 @@ FP32_PEAK_TFLOPS = 157.3
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="dam3d",
+                    choices=["dam3d", "cube-uniform", "cube-clustered"],
+                    help="dam3d: the 3D dam break step (default); cube-*: neighbour search + "
+                         "summation density on 2^24 particles in the unit cube")
+    ap.add_argument("--cube-n", type=int, default=1 << 24)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -299,6 +304,148 @@ def cpu_baseline(info):
     return base
 
 
+def cube_case(kind, n, cells_per_radius=2, seed=0):
+    """Unit-cube set of the neighbour/density microbenchmark: FP32, m = 1/n,
+    h for 50 neighbours on average, binned on the fixed domain [0,1]^3."""
+    import pysph_amd.base.api as base
+    import pysph_amd.sph.api as sph
+    from pysph_amd.solver import generators as gen
+    pa = gen.cube_array(n, kind=kind, seed=seed, precision="single")
+    particles = base.Particles([pa], update_particles=False)
+    cm = particles.cell_manager
+    cm.set_fixed_domain((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))
+    cm.check_jumps = False
+    if cells_per_radius != 1:
+        cm.set_cells_per_radius(cells_per_radius)
+    calc = sph.SPHCalc(particles, [pa], pa, base.CubicSplineKernel(dim=3),
+                       [sph.SPHRho.withargs().get_func(pa, pa)], updates=["rho"])
+    return pa, particles, calc
+
+
+def cube_check(pa, particles, nsample=4096, seed=1):
+    """Neighbour sets of nsample random destinations (queried against the
+    whole set on the GPU) compared with the FP64 oracle, and the CPU
+    baseline: the oracle's summation density over all particles of the
+    sub-cube [0, 1/4)^3, single thread."""
+    import pysph_amd.base.api as base
+    from pysph_amd.base import nnps
+    from oracle import oracle as O
+    cm = particles.cell_manager
+    cm.update()
+    x, y, z, h = (pa.get(f).astype(np.float64) for f in ("x", "y", "z", "h"))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    pick = np.sort(rng.choice(len(x), nsample, replace=False))
+    probe = base.get_particle_array(cl_precision="single", name="probe", x=x[pick],
+                                    y=y[pick], z=z[pick], h=h[pick])
+    cm.bin_array(probe, cm.grid)
+    off_g, nb_g = nnps.neighbor_csr(cm, pa, probe, 2.0)
+    src_o = O.make_array(x=x, y=y, z=z, h=h, m=pa.get("m").astype(np.float64))
+    dst_o = O.make_array(x=x[pick], y=y[pick], z=z[pick], h=h[pick],
+                         m=np.zeros(nsample))
+    off_o, nb_o = O.nnps_csr(dst_o, src_o, cell_size=2.0 * float(h.max()))
+    exact = bool(np.array_equal(off_g, off_o)) and all(
+        np.array_equal(np.sort(nb_g[off_g[i]:off_g[i + 1]]), np.sort(nb_o[off_o[i]:off_o[i + 1]]))
+        for i in range(nsample))
+    sel = (x < 0.25) & (y < 0.25) & (z < 0.25)
+    sub = O.make_array(x=x[sel], y=y[sel], z=z[sel], h=h[sel],
+                       m=pa.get("m").astype(np.float64)[sel])
+    off_s, _ = O.nnps_csr(sub, None, cell_size=2.0 * float(h.max()))
+    O.set_threads(1)
+    # repeat the sample until about 10 s of CPU work
+    reps, total = 0, 0.0
+    while total < 10.0 and reps < 20:
+        t0 = time.perf_counter()
+        O.sphrho(3, sub, [sub], cell_size=2.0 * float(h.max()))
+        total += time.perf_counter() - t0
+        reps += 1
+    t = total / reps
+    return ({"destinations": nsample, "bit_exact": exact,
+             "pairs_checked": int(off_o[-1])},
+            {"value": float(off_s[-1]) / t, "unit": "pair-interactions/s", "cores": 1,
+             "kind": "port",
+             "sample": "summation density (cell build, neighbour search, sums) of the %d "
+                       "particles in [0,1/4)^3 with the FP64 oracle, single thread, %.2f s per "
+                       "evaluation, %d evaluations" % (int(sel.sum()), t, reps)})
+
+
+def main_cube(args):
+    import torch
+    from pysph_amd import _abi
+    kind = args.workload.split("-", 1)[1]
+    pa, particles, calc = cube_case(kind, args.cube_n, args.cells_per_radius)
+    n = pa.get_number_of_particles()
+    cm = particles.cell_manager
+
+    def step():
+        pa.set_dirty(True)
+        calc.sph("rho")
+    step()
+    torch.cuda.synchronize()
+    cnt = torch.empty(n, dtype=torch.int64, device=pa.device)
+    _abi.call("sph_neighbor_counts", torch.cuda.current_stream().cuda_stream, cm.grid,
+              pa.view(), pa.view(), 2.0, cnt.data_ptr())
+    pairs = int(cnt.sum().item())
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(args.steps):
+        step()
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    # the density launch alone, 20 back-to-back (idempotent: rho from positions)
+    from pysph_amd.sph import _ops
+    prm = _ops.make_params(3, radius_scale=2.0)
+    ov = _ops.packed_override([pa])
+    rho = pa.get_carray("rho")
+
+    def launch():
+        _ops.density(cm, prm, pa, [pa], rho, override=ov)
+    launch()
+    torch.cuda.synchronize()
+    e2 = torch.cuda.Event(enable_timing=True)
+    e3 = torch.cuda.Event(enable_timing=True)
+    e2.record()
+    for _ in range(20):
+        launch()
+    e3.record()
+    torch.cuda.synchronize()
+    dens_ms = e2.elapsed_time(e3) / 20
+    # algorithmic bytes: density launch reads x,y,z,h,m and writes rho (24 N);
+    # a step adds the binning (16 N: key and permutation written and read)
+    dens_bytes = 24.0 * n
+    achieved = dens_bytes / (dens_ms * 1e-3) / 1e9
+    out = {
+        "metric": "pair-interactions/s",
+        "value": pairs * args.steps / wall,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "gpu_ms_per_step": e0.elapsed_time(e1) / args.steps,
+        "particles_per_s": n * args.steps / wall,
+        "step_hbm_gbs": 40.0 * n * args.steps / wall / 1e9,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "unit-cube neighbour search + summation density, %s" % kind,
+                   "n": n, "h": float(pa.get_carray("h")[0].item()),
+                   "pairs_per_step": pairs, "mean_neighbours": pairs / n,
+                   "cells_per_radius": args.cells_per_radius,
+                   "step": "re-bin (fixed domain) + pack + density", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "sph::density_scan<float, false, false>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": dens_bytes, "avg_launch_ms": dens_ms,
+                     "launches_timed": 20},
+    }
+    if not args.no_cpu:
+        out["nbr_check"], out["cpu_baseline"] = cube_check(pa, particles)
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -314,6 +461,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
         return
+    if args.workload != "dam3d":
+        return main_cube(args)
     from pysph_amd.sph import _ops
     from pysph_amd import _abi
     _ops.DEFAULT_PAIR_PREC = _abi.SPH_PAIR_F64 if args.pair_prec == "f64" else _abi.SPH_PAIR_NATIVE

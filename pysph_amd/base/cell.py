@@ -122,6 +122,17 @@ class CellManager:
             self.initialize()
             self.grid = None
 
+    def set_cells_per_radius(self, k):
+        """Cell size = max_radius_scale * max h / k (k = 1 is the reference's
+        rule).  Smaller cells test fewer candidates per neighbour at the
+        price of more stencil rows; neighbour sets do not change."""
+        if self.max_h <= 0:
+            self.compute_cell_size(self.min_cell_size, self.max_cell_size)
+        self.min_cell_size = self.max_radius_scale * self.max_h / float(k)
+        self.initialize()
+        self.grid = None
+        return self.cell_size
+
     def set_fixed_domain(self, lo, hi):
         """Bin on a fixed box (no bounds read-back): needed for graph capture.
         Particles outside it are clamped into border cells, which keeps the

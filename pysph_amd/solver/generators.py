@@ -146,7 +146,8 @@ def cube_points(n, kind="uniform", seed=0, n_clusters=64, sigma=0.02, cluster_fr
     cl = centres[which] + rng.normal(0.0, sigma, (nc, 3))
     un = rng.random((n - nc, 3))
     pts = np.concatenate([cl, un]).T
-    return np.clip(pts, 0.0, np.nextafter(1.0, 0.0))
+    # 1 - 2^-24 is the largest value below 1 in FP32 and FP64 alike
+    return np.clip(pts, 0.0, 1.0 - 2.0 ** -24)
 
 
 def cube_array(n, kind="uniform", seed=0, neighbours=50, precision="single"):

@@ -106,15 +106,8 @@ class WCSPHPlan:
         return self.use_list
 
     def set_cells_per_radius(self, k):
-        """Cell size = search radius / k (k = 1 is the reference's 2 max h).
-        Smaller cells test fewer candidates per neighbour at the price of
-        more stencil rows (ring k).  Neighbour sets do not change."""
-        cm = self.cm
-        if cm.max_h <= 0:
-            cm.compute_cell_size(cm.min_cell_size, cm.max_cell_size)
-        cm.min_cell_size = cm.max_radius_scale * cm.max_h / float(k)
-        cm.initialize()
-        cm.grid = None
+        """Cell size = search radius / k for the shared CellManager."""
+        self.cm.set_cells_per_radius(k)
         self.cells_per_radius = k
 
     # ----------------------------------------------------------------- match

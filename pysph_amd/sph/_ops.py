@@ -58,11 +58,43 @@ def check_same_dtype(arrays):
         raise TypeError("all arrays of one operation must share a dtype")
 
 
+# FP32 density passes called without packed inputs pack (x,y,z,h) and
+# (u,v,w,m) into per-array float4 scratch first and take the fast path.
+AUTO_PACK = True
+
+
+def packed_override(arrays, override=None):
+    """`override` extended with freshly packed pos4/vel4 copies of every
+    FP32 array that has none (the copies follow the current slot order)."""
+    ov = dict(override or {})
+    for pa in arrays:
+        if pa.dtype != torch.float32 or "pos4" in ov.get(id(pa), {}):
+            continue
+        n = pa.get_number_of_particles()
+        sc = pa.__dict__.get("_pack_scratch")
+        if sc is None or sc["pos4"].shape[0] < max(n, 1):
+            sc = {f: torch.empty((max(n, 1), 4), dtype=torch.float32, device=pa.device)
+                  for f in ("pos4", "vel4")}
+            pa.__dict__["_pack_scratch"] = sc
+        # pack what the pass would read: the array's fields as overridden
+        v = pa.view()
+        for f, t in ov.get(id(pa), {}).items():
+            setattr(v, f, ptr(t))
+        _abi.call("sph_pack", require_gpu(sc["pos4"]), v, sc["pos4"].data_ptr(),
+                  sc["vel4"].data_ptr())
+        d = dict(ov.get(id(pa), {}))
+        d.update(sc)
+        ov[id(pa)] = d
+    return ov
+
+
 def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, nl=None):
     """SPHRho over srcs into rho (+ Tait into p, cs); `nl` (SphNlist)
     receives the accepted neighbours for a following momentum pass."""
     check_same_dtype([dst] + list(srcs))
     stream = require_gpu(rho)
+    if AUTO_PACK and dst.dtype == torch.float32 and nl is None:
+        override = packed_override([dst] + [s for s in srcs if s is not dst], override)
     vd = views([dst], override)
     vs = views(srcs, override)
     _abi.call("sph_density", stream, cm.grid, prm, vd, len(srcs), vs,
