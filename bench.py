@@ -25,9 +25,13 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dam3d",
-                    choices=["dam3d", "cube-uniform", "cube-clustered"],
+                    choices=["dam3d", "cube-uniform", "cube-clustered", "batch2d"],
                     help="dam3d: the 3D dam break step (default); cube-*: neighbour search + "
-                         "summation density on 2^24 particles in the unit cube")
+                         "summation density on 2^24 particles in the unit cube; batch2d: 512 "
+                         "2D scenes (drops and dam breaks) stepped together")
+    ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--no-graph", action="store_true", help="batch2d: launch eagerly")
+    ap.add_argument("--no-autotune", action="store_true", help="batch2d: default schedule")
     ap.add_argument("--cube-n", type=int, default=1 << 24)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
@@ -446,6 +450,143 @@ def main_cube(args):
     print(json.dumps(out), flush=True)
 
 
+def batch_pairs(sc, cells_per_radius=2):
+    """Pairs per RK2 step of a list of (fluid, boundary, info) scenes: the
+    density pass of every array from all arrays and the momentum pass of the
+    fluid, twice (both stages), counted on the GPU per scene."""
+    import torch
+    import pysph_amd.base.api as base
+    from pysph_amd import _abi
+    total = 0
+    stream = torch.cuda.current_stream().cuda_stream
+    for f, b, info in sc:
+        arrays = [f] + ([b] if b is not None else [])
+        cm = base.CellManager(arrays_to_bin=arrays)
+        cm.set_fixed_domain(*info["domain"])
+        cm.check_jumps = False
+        cm.set_cells_per_radius(cells_per_radius)
+        cm.update()
+        p = 0
+        for dst in arrays:
+            for src in arrays:
+                n = dst.get_number_of_particles()
+                c = torch.empty(max(n, 1), dtype=torch.int64, device=dst.device)
+                _abi.call("sph_neighbor_counts", stream, cm.grid, dst.view(), src.view(), 2.0,
+                          c.data_ptr())
+                k = int(c[:n].sum().item())
+                p += k
+                if dst is f:
+                    p += k  # the momentum pass visits the same pairs
+        total += 2 * p
+    return total
+
+
+def main_batch(args):
+    import torch
+    from pysph_amd.batch.engine import BatchedWCSPH, scene_from_arrays
+    from pysph_amd.solver import generators as gen
+    sc = gen.batch_scenes(args.scenes, seed=0)
+    nsc = len(sc)
+    pairs = batch_pairs(gen.batch_scenes(args.scenes, seed=0), args.cells_per_radius)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        eng = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc],
+                           cells_per_radius=args.cells_per_radius)
+        tune = None if args.no_autotune else eng.autotune(reps=3)
+        if not args.no_graph:
+            eng.capture()
+        for _ in range(args.warmup):
+            eng.step()
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record()
+        for _ in range(args.steps):
+            eng.step()
+        e1.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        dens_ms = eng.time_phase("density", eng.sched["density"], reps=20)
+        stage_ms = eng.time_phase("stage", eng.sched["stage"], reps=20)
+    nf, nb = eng.nf, eng.nb
+    # algorithmic bytes (2D FP32): density on fluid destinations reads x,y,h,m
+    # of both sets and writes rho,p,cs of the fluid; stage 1 reads
+    # x,y,u,v,h,m,rho,p,cs of both sets and writes k1 (4) and X1 (4) of the fluid
+    kb = {"density": 28.0 * nf + 16.0 * nb, "stage": 68.0 * nf + 36.0 * nb}
+    per_step = {"density": 2 * dens_ms, "stage": 2 * stage_ms}
+    dom = max(per_step, key=per_step.get)
+    dom_ms = dens_ms if dom == "density" else stage_ms
+    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
+    kinds = [i["kind"] for _, _, i in sc]
+    out = {
+        "metric": "pair-interactions/s",
+        "value": pairs * args.steps / wall,
+        "unit": "pair-interactions/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "gpu_ms_per_step": e0.elapsed_time(e1) / args.steps,
+        "scene_steps_per_s": nsc * args.steps / wall,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched 2D scenes: %d elliptical drops + %d dam breaks, RK2"
+                               % (kinds.count("drop"), kinds.count("dam")),
+                   "scenes": nsc, "n_fluid": nf, "n_boundary": nb, "pairs_per_step": pairs,
+                   "graph": not args.no_graph, "cells_per_radius": args.cells_per_radius,
+                   "schedules": eng.sched, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm",
+                     "kernel": "sph::batch_density_kernel" if dom == "density"
+                     else "sph::batch_stage_kernel<1>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kb[dom], "avg_launch_ms": dom_ms, "launches_timed": 20},
+        "phase_ms": {"density_fluid": dens_ms, "stage1": stage_ms},
+    }
+    if tune is not None:
+        out["autotune"] = {ph: {"best": r[0], "ms": r[1],
+                                "default_ms": next(t for c, t in r[2] if c["mode"] == 0)}
+                           for ph, r in tune.items()}
+    eng.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = batch_cpu_baseline(sc)
+    print(json.dumps(out), flush=True)
+
+
+def batch_cpu_baseline(sc):
+    """RK2 steps of the FP64 oracle over the batch's smallest scenes in turn
+    (single thread) until about 10 s of CPU work."""
+    from oracle import oracle as O
+    from tests.util import to_oracle
+    from pysph_amd.solver import generators as gen
+    order = sorted(range(len(sc)), key=lambda k: sc[k][0].get_number_of_particles())
+    fresh = gen.batch_scenes(len(sc), seed=0)
+    O.set_threads(1)
+    total_t, total_pairs, used = 0.0, 0, 0
+    for k in order:
+        f, b, info = fresh[k]
+        arrs = [to_oracle(f, 0)] + ([to_oracle(b, 1)] if b is not None else [])
+        cs = 2.0 * info["h"]
+        p = 0
+        for dst in arrs:
+            for src in arrs:
+                off, _ = O.nnps_csr(dst, src if src is not dst else None, cell_size=cs)
+                p += int(off[-1]) * (2 if dst is arrs[0] else 1)
+        sch = O.scheme(2, integrator="rk2", dt=info["dt"], co=info["co"], ro=info["rho0"],
+                       gamma=7.0, alpha=info["alpha"], beta=info["beta"], eta=info["eta"],
+                       gravity=info["g"], xsph_eps=info["eps"] or None, cell_size=cs)
+        t0 = time.perf_counter()
+        O.scheme_step(sch, arrs, 1)
+        total_t += time.perf_counter() - t0
+        total_pairs += 2 * p
+        used += 1
+        if total_t > 10.0:
+            break
+    return {"value": total_pairs / total_t, "unit": "pair-interactions/s", "cores": 1,
+            "kind": "port",
+            "sample": "one RK2 step of each of the %d smallest scenes with the FP64 oracle, "
+                      "single thread, %.1f s" % (used, total_t)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -461,6 +602,8 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes": a.numel() * 4, "copies": 3}), flush=True)
         return
+    if args.workload == "batch2d":
+        return main_batch(args)
     if args.workload != "dam3d":
         return main_cube(args)
     from pysph_amd.sph import _ops

@@ -270,6 +270,94 @@ int32_t sph_conservation_f64(void *stream, const sph_params *prm, const sph_view
 /* Fill n int32 words with a value (graph-capturable, no hipMemset). */
 int32_t sph_fill_i32(void *stream, int32_t *dst_dev, int64_t n, int32_t value);
 
+/* ---- batched multi-scene WCSPH ------------------------------------------
+ * No counterpart in the reference, which steps one Particles set at a time
+ * (solver.py:457-508).  Many independent scenes share one set of device
+ * buffers: every scene's fluid particles are one slice of a concatenated
+ * fluid set, its boundary particles one slice of a boundary set.  Each scene
+ * has its own grid in one global cell space (cells [cell_base, cell_base +
+ * nx*ny*nz)), so one counting sort bins all scenes and keeps them
+ * contiguous.  The per-particle arithmetic is that of sph_density and
+ * sph_wcsph_stage (packed FP32 path, RK2), so a scene stepped in a batch
+ * gives the bits it gives when stepped alone with the same grid. */
+typedef struct sph_batch_scene {
+    int64_t f0, nf;     /* fluid slice [f0, f0 + nf) */
+    int64_t b0, nb;     /* boundary slice (nb = 0: no boundary) */
+    sph_grid grid;      /* cell grid of the scene */
+    sph_params prm;     /* dim, radius_scale, Tait, viscosity, XSPH, gravity */
+    double rmax;        /* radius_scale * max h of the scene */
+    double dt;          /* RK2 time step */
+    double cost;        /* relative cost per particle (schedule order) */
+} sph_batch_scene;
+
+/* One concatenated particle set (FP32 SoA; uid = original index, carried
+ * along by every binning). */
+typedef struct sph_batch_set {
+    float *x, *y, *z, *u, *v, *w, *h, *m, *rho, *p, *cs;
+    int32_t *uid;
+} sph_batch_set;
+
+/* Device buffers of a batch, owned by the caller and fixed for the life of
+ * the handle (so that a captured step stays valid).  The fluid state
+ * alternates between fa (step start/end) and fb (stage 1). */
+typedef struct sph_batch_buffers {
+    int64_t nf_total, nb_total;
+    sph_batch_set fa, fb;        /* fluid sets */
+    sph_batch_set bnd, bnd_in;   /* boundary set (binned) and its input order */
+    float *x1a[6], *x1b[6];      /* RK2 stage state X1, before / after its binning */
+    float *k1a[6], *k1b[6];      /* RK2 k1 (momentum 3, XSPH 3), likewise */
+    float *fpos4, *fvel4, *faux4;  /* packed fluid copies (float4 per particle) */
+    float *bpos4, *bvel4, *baux4;  /* packed boundary copies */
+    int32_t *fscene, *bscene;    /* scene of each slot */
+    int32_t *fperm, *fcell, *fcell_start;  /* fluid binning */
+    int32_t *bperm, *bcell, *bcell_start;  /* boundary binning */
+    void *ws;                    /* binning workspace */
+    size_t ws_bytes;             /* >= sph_bin_workspace_bytes(max(nf, nb), ncells) */
+} sph_batch_buffers;
+
+/* Work schedule of a neighbour phase (never changes results).  Tiles are 64
+ * consecutive particles of one scene, one wave each.
+ *   mode 0: one wave per tile (a plain grid);
+ *   mode 1: persistent: blocks_per_cu blocks per CU, each XCD walks its own
+ *           segment of the tile list in a fixed stride;
+ *   mode 2: persistent, waves take `grab` tiles at a time from their XCD's
+ *           segment with an atomic counter.
+ *   order 0: tiles in scene order, XCD segments of equal tile count;
+ *   order 1: XCD segments of equal estimated cost (scene cost x particles),
+ *           most expensive tiles first within a segment. */
+typedef struct sph_batch_sched {
+    int32_t mode;
+    int32_t blocks_per_cu;
+    int32_t grab;
+    int32_t order;
+} sph_batch_sched;
+
+/* Number of cells of the global cell space for these scenes. */
+int64_t sph_batch_ncells(int32_t nscenes, const sph_batch_scene *scenes_host);
+int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene *scenes_host,
+                         const sph_batch_buffers *buffers_host, void **handle);
+int32_t sph_batch_destroy(void *handle);
+/* which 0: fluid by X (fa -> fb); 1: fluid by X1 (fb -> fa, x1a -> x1b,
+ * k1a -> k1b); 2: boundary (bnd_in -> bnd). */
+int32_t sph_batch_bin(void *handle, void *stream, int32_t which);
+/* which 0: fluid from fb; 1: fluid from X1 (x1b, components >= dim and h, m
+ * from fa); 2: boundary. */
+int32_t sph_batch_pack(void *handle, void *stream, int32_t which);
+/* SPHRho + TaitEquation of every scene: phase 0 on fb (stage 1), phase 1
+ * on fa with X1 positions (stage 2); set 0 fluid destinations, 1 boundary. */
+int32_t sph_batch_density(void *handle, void *stream, int32_t phase, int32_t set,
+                          const sph_batch_sched *sched);
+/* RK2 stage of every scene's fluid: 1 = predictor (fb -> x1a, k1a),
+ * 2 = final update of fa in place. */
+int32_t sph_batch_stage(void *handle, void *stream, int32_t stage,
+                        const sph_batch_sched *sched);
+
+/* ---- stream capture into HIP graphs (any sequence of the calls above) -- */
+int32_t sph_graph_begin(void *stream);
+int32_t sph_graph_end(void *stream, void **graph_exec);
+int32_t sph_graph_launch(void *graph_exec, void *stream);
+int32_t sph_graph_destroy(void *graph_exec);
+
 #ifdef __cplusplus
 }
 #endif

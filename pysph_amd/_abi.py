@@ -60,6 +60,34 @@ class SphUpdateOp(ctypes.Structure):
     _fields_ = [("out", _vp), ("a", _vp), ("b", _vp), ("c", _vp)]
 
 
+class SphBatchScene(ctypes.Structure):
+    _fields_ = [("f0", ctypes.c_int64), ("nf", ctypes.c_int64), ("b0", ctypes.c_int64),
+                ("nb", ctypes.c_int64), ("grid", SphGrid), ("prm", SphParams),
+                ("rmax", ctypes.c_double), ("dt", ctypes.c_double), ("cost", ctypes.c_double)]
+
+
+BATCH_FIELDS = ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs")
+
+
+class SphBatchSet(ctypes.Structure):
+    _fields_ = [(f, _vp) for f in BATCH_FIELDS] + [("uid", _vp)]
+
+
+class SphBatchBuffers(ctypes.Structure):
+    _fields_ = [("nf_total", ctypes.c_int64), ("nb_total", ctypes.c_int64),
+                ("fa", SphBatchSet), ("fb", SphBatchSet), ("bnd", SphBatchSet),
+                ("bnd_in", SphBatchSet),
+                ("x1a", _vp * 6), ("x1b", _vp * 6), ("k1a", _vp * 6), ("k1b", _vp * 6)] + [
+        (f, _vp) for f in ("fpos4", "fvel4", "faux4", "bpos4", "bvel4", "baux4", "fscene",
+                           "bscene", "fperm", "fcell", "fcell_start", "bperm", "bcell",
+                           "bcell_start", "ws")] + [("ws_bytes", ctypes.c_size_t)]
+
+
+class SphBatchSched(ctypes.Structure):
+    _fields_ = [("mode", ctypes.c_int32), ("blocks_per_cu", ctypes.c_int32),
+                ("grab", ctypes.c_int32), ("order", ctypes.c_int32)]
+
+
 class SphError(RuntimeError):
     """A non-zero return code of the C ABI."""
 
@@ -110,6 +138,19 @@ _SIGS = {
     "sph_conservation_f64": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp]),
     "sph_fill_i32": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32]),
     "sph_pack": (ctypes.c_int32, [_vp, _P(SphView), _vp, _vp]),
+    "sph_batch_ncells": (ctypes.c_int64, [ctypes.c_int32, _P(SphBatchScene)]),
+    "sph_batch_create": (ctypes.c_int32, [ctypes.c_int32, _P(SphBatchScene),
+                                          _P(SphBatchBuffers), _P(_vp)]),
+    "sph_batch_destroy": (ctypes.c_int32, [_vp]),
+    "sph_batch_bin": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32]),
+    "sph_batch_pack": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32]),
+    "sph_batch_density": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                           _P(SphBatchSched)]),
+    "sph_batch_stage": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, _P(SphBatchSched)]),
+    "sph_graph_begin": (ctypes.c_int32, [_vp]),
+    "sph_graph_end": (ctypes.c_int32, [_vp, _P(_vp)]),
+    "sph_graph_launch": (ctypes.c_int32, [_vp, _vp]),
+    "sph_graph_destroy": (ctypes.c_int32, [_vp]),
 }
 
 EXPORTS = tuple(_SIGS)

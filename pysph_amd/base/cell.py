@@ -51,6 +51,25 @@ def construct_immediate_neighbor_list(cell_id, include_self=True, distance=1):
     return out
 
 
+def grid_for_domain(lo, hi, cell_size, slack=2):
+    """Cell grid covering [lo, hi] with `slack` spare cells on each side.
+    An axis on which the domain has no extent (z of a 2D set) gets a single
+    layer of cells, so that 2D stencils do not visit empty planes."""
+    cs = float(cell_size)
+    g = _abi.SphGrid()
+    g.cell_size = cs
+    for d in range(3):
+        s = 0 if lo[d] == hi[d] else slack
+        c0 = math.floor(lo[d] / cs) - s
+        c1 = math.floor(hi[d] / cs) + s
+        g.lo[d] = int(max(min(c0, 2**30), -2**30))
+        g.dims[d] = int(max(1, min(c1 - c0 + 1, 2**20)))
+    ncells = g.dims[0] * g.dims[1] * g.dims[2]
+    if ncells > 2**31 - 2:
+        raise MemoryError("cell grid of %d cells is too large" % ncells)
+    return g
+
+
 class CellManager:
     """Bins every array of `arrays_to_bin` on one shared grid."""
 
@@ -171,19 +190,7 @@ class CellManager:
         return lo, hi
 
     def make_grid(self, lo, hi):
-        cs = self.cell_size
-        g = _abi.SphGrid()
-        g.cell_size = cs
-        s = self.grid_slack
-        for d in range(3):
-            c0 = math.floor(lo[d] / cs) - s
-            c1 = math.floor(hi[d] / cs) + s
-            g.lo[d] = int(max(min(c0, 2**30), -2**30))
-            g.dims[d] = int(max(1, min(c1 - c0 + 1, 2**20)))
-        ncells = g.dims[0] * g.dims[1] * g.dims[2]
-        if ncells > 2**31 - 2:
-            raise MemoryError("cell grid of %d cells is too large" % ncells)
-        return g
+        return grid_for_domain(lo, hi, self.cell_size, self.grid_slack)
 
     @staticmethod
     def grid_key(g):

@@ -11,7 +11,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsph_mi355x.so")
-SOURCES = ["sph_abi.hip", "sph_bin.hip", "sph_passes.hip", "sph_fast.hip", "sph_integrate.hip"]
+SOURCES = ["sph_abi.hip", "sph_bin.hip", "sph_passes.hip", "sph_fast.hip", "sph_integrate.hip",
+           "sph_batch.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

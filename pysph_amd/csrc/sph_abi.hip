@@ -84,6 +84,7 @@ Grid make_grid(const sph_grid& g, double rmax) {
   G.rmax = rmax;
   int r = (int)ceil(rmax / g.cell_size);
   G.ring0 = r < 1 ? 1 : r;
+  G.cell_base = 0;
   return G;
 }
 

@@ -130,7 +130,7 @@ __global__ void fill_i32_kernel(int32_t* __restrict__ p, int64_t n, int32_t v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
-static int32_t fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
+int32_t fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v) {
   if (n <= 0) return SPH_OK;
   int nb = (int)((n + kBlock - 1) / kBlock);
   if (nb > 65536) nb = 65536;
@@ -220,6 +220,27 @@ __global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
     const int py = t % g.dims[1], pz = t / g.dims[1];
     if (abs(px - cx) > 1 || abs(py - cy) > 1 || abs(pz - cz) > 1) atomicAdd(jumps, 1);
   }
+}
+
+// Multi-scene binning: particle i belongs to scene scene[i], whose grid
+// occupies cells [cell_base, cell_base + nx*ny*nz) of one global cell space,
+// so one counting sort bins every scene and keeps scenes contiguous.
+__global__ __launch_bounds__(kBlock) void batch_bin_count(int n, const float* __restrict__ x,
+                                                          const float* __restrict__ y,
+                                                          const float* __restrict__ z,
+                                                          const int* __restrict__ scene,
+                                                          const BatchBinGrid* __restrict__ grids,
+                                                          int* __restrict__ cell_tmp,
+                                                          int* __restrict__ counts) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const BatchBinGrid g = grids[scene[i]];
+  const int cx = grid_coord((double)x[i], g.cs, g.lo[0], g.dims[0]);
+  const int cy = grid_coord((double)y[i], g.cs, g.lo[1], g.dims[1]);
+  const int cz = grid_coord((double)z[i], g.cs, g.lo[2], g.dims[2]);
+  const int c = g.cell_base + (cz * g.dims[1] + cy) * g.dims[0] + cx;
+  cell_tmp[i] = c;
+  atomicAdd(&counts[c], 1);
 }
 
 __global__ __launch_bounds__(kBlock) void bin_scatter(int n, const int* __restrict__ cell_tmp,
@@ -352,6 +373,45 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
 }  // namespace sph
 
 using namespace sph;
+
+namespace sph {
+
+// Binning of a multi-scene particle set (see batch_bin_count); outputs as
+// sph_bin: perm (new slot -> old slot), cell (global cell id per new slot),
+// cell_start (ncells + 1).  The workspace is sph_bin_workspace_bytes(n, ncells).
+int32_t batch_bin(hipStream_t st, int64_t n, int64_t ncells, const float* x, const float* y,
+                  const float* z, const int* scene, const BatchBinGrid* grids, int32_t* perm,
+                  int32_t* cell, int32_t* cell_start, void* ws) {
+  char* w = (char*)ws;
+  int* counts = (int*)w;
+  w += (size_t)(ncells + 64) * 4;
+  int* cell_tmp = (int*)w;
+  w += (size_t)(n + 64) * 4;
+  int* bucket = (int*)w;
+  w += (size_t)(n + 64) * 4;
+  int* partial = (int*)w;
+  SPH_TRY(fill_i32(st, counts, ncells, 0));
+  const int nb = nblocks_for(n);
+  if (n > 0)
+    hipLaunchKernelGGL(batch_bin_count, dim3(nb), dim3(kBlock), 0, st, (int)n, x, y, z, scene,
+                       grids, cell_tmp, counts);
+  SPH_TRY(launch_status("batch_bin/count"));
+  SPH_TRY((exclusive_scan<int, int>(st, counts, ncells, cell_start, partial)));
+  if (n == 0) return SPH_OK;
+  SPH_TRY(fill_i32(st, counts, ncells, 0));
+  hipLaunchKernelGGL(bin_scatter, dim3(nb), dim3(kBlock), 0, st, (int)n, cell_tmp, cell_start,
+                     counts, bucket);
+  hipLaunchKernelGGL(bin_rank, dim3(nb), dim3(kBlock), 0, st, (int)n, cell_tmp, cell_start,
+                     bucket, perm, cell);
+  return launch_status("batch_bin");
+}
+
+int32_t permute_fields(hipStream_t st, int64_t n, const int32_t* perm, int32_t nfields,
+                       const void* const* src, void* const* dst, const int32_t* esz) {
+  return sph_permute(st, n, perm, nfields, src, dst, esz);
+}
+
+}  // namespace sph
 
 extern "C" {
 

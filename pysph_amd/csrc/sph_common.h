@@ -33,6 +33,7 @@ struct Grid {
   float inv_cs_f;  // 1 / cs_f (row culling only; conservative by its margin)
   double rmax;   // largest search radius of the call (radius_scale * max h)
   int ring0;     // ceil(rmax / cs)
+  int cell_base; // cell ids stored per particle are cell_base + local id
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
@@ -114,6 +115,7 @@ struct Within32 {
 
 // Cell coordinate decode of a linear cell id (x fastest).
 __device__ __forceinline__ void cell_xyz(int c, const Grid& g, int& cx, int& cy, int& cz) {
+  c -= g.cell_base;
   cx = c % g.dims[0];
   const int t = c / g.dims[0];
   cy = t % g.dims[1];

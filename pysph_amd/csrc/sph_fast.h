@@ -1,0 +1,471 @@
+// Device building blocks of the packed FP32 neighbour passes, shared by the
+// single-array kernels (sph_fast.hip) and the batched multi-scene kernels
+// (sph_batch.hip).
+//
+// Packed FP32 fast path of the neighbour passes.
+//
+// Inputs are float4 copies per particle: pos4 = (x,y,z,h), vel4 = (u,v,w,m),
+// aux4 = (rho,p,cs,p/rho^2).  One lane per destination particle, the
+// particles sorted by cell so that the lanes of a wave share stencil rows.
+//
+// Cell scans: a lane screens the candidates of its stencil rows (one 16-byte
+// load each, four in flight) and queues the accepted source indices in LDS;
+// when a queue is full, and after each source array, the wave drains all
+// queues in one wave-uniform loop, so the pair arithmetic runs with nearly
+// all lanes active instead of the ~15% that accept any one candidate.
+//
+// Neighbour lists: a density scan can record the accepted sources of every
+// destination.  Without skin the list holds exactly the neighbours and the
+// momentum pass of the same stage reads it without testing again.  With a
+// skin the scan records every source within radius*(1+skin) (a superset):
+// later passes apply the exact test to each entry, as long as no particle
+// has moved more than skin*radius/2 (checked on the device; lanes where it
+// fails search the cells again with a widened ring).
+//
+// Reference formulas: SPHRho (density_funcs.pyx:38-84), TaitEquation
+// (eos_funcs.pyx:105-116), MomentumEquation (pressure_funcs.pyx:154-245),
+// XSPHCorrection (xsph_funcs.pyx:31-88), cubic spline (kernels.pyx:345-449),
+// integrator updates (integrator.py:472-514, :597-619).
+#pragma once
+#include <math.h>
+
+#include "sph_passes.h"
+
+namespace sph {
+
+// LDS queue depth per lane and candidate loads in flight per lane
+#ifndef SPH_QUEUE
+#define SPH_QUEUE 20
+#endif
+#ifndef SPH_UNROLL
+#define SPH_UNROLL 4
+#endif
+constexpr int kQueue = SPH_QUEUE;
+constexpr int kUnroll = SPH_UNROLL;
+__device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Exact neighbour test (FP32 screen, FP64 decision in the band).
+__device__ __forceinline__ bool accept_exact(const float4& s, const float4& d, float r2f,
+                                             double r2d) {
+  const float dx = s.x - d.x, dy = s.y - d.y, dz = s.z - d.z;
+  const float d2f = dx * dx + dy * dy + dz * dz;
+  return Within32::test(s.x, s.y, s.z, d.x, d.y, d.z, d2f, r2f, r2d);
+}
+
+// Superset test of the list build: inside radius*(1+skin), with a margin
+// far above FP32 rounding.
+__device__ __forceinline__ bool accept_super(const float4& s, const float4& d, float rl2f) {
+  const float dx = s.x - d.x, dy = s.y - d.y, dz = s.z - d.z;
+  return dx * dx + dy * dy + dz * dz < rl2f;
+}
+
+template <typename CT>
+__device__ __forceinline__ CT pair_d2(const float4& d, const float4& s, CT& rx, CT& ry, CT& rz) {
+  rx = (CT)d.x - (CT)s.x;
+  ry = (CT)d.y - (CT)s.y;
+  rz = (CT)d.z - (CT)s.z;
+  return rx * rx + ry * ry + rz * rz;
+}
+
+// Reciprocal and square root: hardware 1-ulp instructions for FP32 pair
+// arithmetic, IEEE operations for FP64.
+template <typename CT>
+__device__ __forceinline__ CT frcp(CT x) {
+  return CT(1) / x;
+}
+template <>
+__device__ __forceinline__ float frcp<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+template <typename CT>
+__device__ __forceinline__ CT fsqrt(CT x) {
+  return sqrt(x);
+}
+template <>
+__device__ __forceinline__ float fsqrt<float>(float x) {
+  return __builtin_amdgcn_sqrtf(x);
+}
+
+// Branch-free cubic spline shape W(q) (kernels.pyx:367-374).
+template <typename CT>
+__device__ __forceinline__ CT w_sel(CT q) {
+  const CT t = CT(2) - q;
+  const CT outer = CT(0.25) * t * t * t;
+  const CT inner = CT(1) - CT(1.5) * (q * q) * (CT(1) - CT(0.5) * q);
+  return q > CT(2) ? CT(0) : (q > CT(1) ? outer : inner);
+}
+
+// Branch-free gradient factor (kernels.pyx:403-412): grad W = r_ab * g.
+template <typename CT>
+__device__ __forceinline__ CT g_sel(CT q, CT r, CT h, CT inv) {
+  const CT t = CT(2) - q;
+  const CT outer = CT(-0.75) * t * t * frcp<CT>(h * r);
+  const CT inner = CT(3) * (CT(0.75) * q - CT(1)) * (inv * inv);
+  return q > CT(2) ? CT(0) : (q >= CT(1) ? outer : (q > CT(1e-14) ? inner : CT(0)));
+}
+
+// ---------------------------------------------------------------------------
+// stencil rows with geometric culling
+// ---------------------------------------------------------------------------
+struct Cull {
+  float xd, yd, zd, rc2;
+};
+
+// Distance from v to the extent of cell c along one axis; border cells
+// reach to infinity because the binning clamps outside particles into them.
+__device__ __forceinline__ float axis_gap(float v, int c, int n, int lo, float cs) {
+  float gap = 0.f;
+  if (c > 0) gap = fmaxf(gap, (float)(c + lo) * cs - v);
+  if (c < n - 1) gap = fmaxf(gap, v - (float)(c + lo + 1) * cs);
+  return gap;
+}
+
+// rc is widened for FP32 rounding of positions and cell boundaries.
+__device__ __forceinline__ Cull make_cull(const float4& pd, double rc, const Grid& g) {
+  Cull c;
+  c.xd = pd.x;
+  c.yd = pd.y;
+  c.zd = pd.z;
+  const float scale = fabsf(pd.x) + fabsf(pd.y) + fabsf(pd.z) + g.cs_f;
+  const float r = (float)rc * 1.00001f + 8e-7f * scale;
+  c.rc2 = r * r;
+  return c;
+}
+
+// Particle range of the cells of grid row (y, z) within the stencil ring of
+// cell column cx whose extent can hold a point within the cull radius.
+__device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restrict__ cstart,
+                                               int cx, int y, int z, int ring, const Cull& cu,
+                                               int& j0, int& j1) {
+  j0 = j1 = 0;
+  if (y < 0 || y >= g.dims[1] || z < 0 || z >= g.dims[2]) return;
+  const float gy = axis_gap(cu.yd, y, g.dims[1], g.lo[1], g.cs_f);
+  const float gz = axis_gap(cu.zd, z, g.dims[2], g.lo[2], g.cs_f);
+  const float w2 = cu.rc2 - gy * gy - gz * gz;
+  if (!(w2 >= 0.f)) return;
+  const float w = sqrtf(w2);
+  const float fa = floorf((cu.xd - w) * g.inv_cs_f) - (float)g.lo[0];
+  const float fb = floorf((cu.xd + w) * g.inv_cs_f) - (float)g.lo[0];
+  int x0 = max(cx - ring, 0), x1 = min(cx + ring, g.dims[0] - 1);
+  if (fa > (float)x1 || fb < (float)x0) return;  // NaN positions keep the ring
+  if (fa > (float)x0) x0 = (int)fa;
+  if (fb < (float)x1) x1 = (int)fb;
+  const int base = (z * g.dims[1] + y) * g.dims[0];
+  j0 = cstart[base + x0];
+  j1 = cstart[base + x1 + 1];
+}
+
+// Whether a list built with the given skin still holds every neighbour of a
+// destination of search radius r after a displacement of at most disp.
+__device__ __forceinline__ bool list_valid(const float* disp, float skin, double r) {
+  if (disp == nullptr) return true;
+  return 2.0 * (double)(*disp) <= (double)skin * r;
+}
+
+// ---------------------------------------------------------------------------
+// density
+// ---------------------------------------------------------------------------
+template <typename CT>
+__device__ __forceinline__ CT w_term(const float4& pd, const float4& ps, CT hd, CT m, int dim) {
+  CT rx, ry, rz;
+  const CT d2 = pair_d2<CT>(pd, ps, rx, ry, rz);
+  const CT hab = CT(0.5) * (hd + (CT)ps.w);
+  const CT inv = frcp<CT>(hab);
+  const CT qq = fsqrt<CT>(d2) * inv;
+  return m * (w_sel<CT>(qq) * h_dim_inv<CT>(inv, dim));
+}
+
+template <bool TAIT>
+__device__ __forceinline__ void density_epilogue(const PassArgs<float>& a, int i, bool local,
+                                                 double rho) {
+  if (!local) {
+    a.rho[i] = 0.f;
+    if (TAIT) {
+      a.p[i] = 0.f;
+      a.cs[i] = 0.f;
+    }
+    if (a.dst.aux4 != nullptr) a.dst.aux4[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    return;
+  }
+  const float rhof = (float)rho;
+  a.rho[i] = rhof;
+  if (TAIT) {
+    double p, cs;
+    tait((double)rhof, a.co, a.ro, a.gamma, a.B, p, cs);
+    const float pf = (float)p;
+    a.p[i] = pf;
+    a.cs[i] = (float)cs;
+    if (a.dst.aux4 != nullptr)
+      a.dst.aux4[i] = make_float4(rhof, pf, (float)cs,
+                                  (float)((double)pf / ((double)rhof * (double)rhof)));
+  }
+}
+
+// Cell scan of destination i (all lanes of the wave call it together; q
+// is the lane's LDS queue, kQueue ints 64 apart); BUILD also records the
+// list (superset radius).
+template <typename CT, bool TAIT, bool BUILD>
+__device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<float>& a,
+                                                  int i, int* q) {
+  const bool valid = i < a.dst.n;
+  const bool local = valid && (a.dst.tag == nullptr || a.dst.tag[i] == 0);
+  const float4 pd = valid ? a.dst.pos4[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+  const double r = (double)pd.w * a.rs;
+  const double r2d = r * r;
+  const float r2f = (float)r2d;
+  // a list built without skin holds exactly the neighbours
+  const bool exact_list = !BUILD || a.nl_skin == 0.f;
+  const double rl = BUILD ? r * (1.0 + (double)a.nl_skin) : r;
+  const float rl2f = (float)(rl * rl) * 1.00001f;
+  const bool write_list = BUILD && local;
+  int cx = 0, cy = 0, cz = 0;
+  if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
+  const int ring = ring_of(g, a.disp);
+  const double dd = a.disp ? (double)*a.disp : 0.0;
+  const Cull cu = make_cull(pd, rl + dd, g);
+  const int dim = a.dim;
+  const CT hd = (CT)pd.w;
+  const int64_t ndst = a.dst.n;
+  CT rho = CT(0);
+  int lpos = 0;
+  for (int s = 0; s < a.nsrc; ++s) {
+    const View<float> S = a.src[s];
+    CT nr = CT(0);
+    int cnt = 0, cnt_s = 0;
+    auto drain = [&]() {
+      const int m = wave_max(cnt);
+      for (int k = 0; k < m; ++k) {
+        if (k < cnt) {
+          const int j = q[k * 64];
+          const float4 ps = S.pos4[j];
+          if (exact_list || accept_exact(ps, pd, r2f, r2d))
+            nr += w_term<CT>(pd, ps, hd, (CT)S.vel4[j].w, dim);
+          if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
+        }
+      }
+      lpos += cnt;
+      cnt_s += cnt;
+      cnt = 0;
+    };
+    for (int dz = -ring; dz <= ring; ++dz) {
+      for (int dy = -ring; dy <= ring; ++dy) {
+        int j0 = 0, j1 = 0;
+        if (local) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
+        while (__any(j0 < j1)) {
+          if (__any(cnt > kQueue - kUnroll)) drain();
+          float4 c[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : kFar4;
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            const bool ok =
+                exact_list ? accept_exact(c[u], pd, r2f, r2d) : accept_super(c[u], pd, rl2f);
+            if (j0 + u < j1 && ok) {
+              q[cnt * 64] = j0 + u;
+              ++cnt;
+            }
+          }
+          j0 += kUnroll;
+        }
+      }
+    }
+    drain();
+    rho += nr;
+    if (BUILD && valid) a.nl_count[(int64_t)s * ndst + i] = local ? cnt_s : 0;
+  }
+  if (!valid) return;
+  density_epilogue<TAIT>(a, i, local, (double)(rho * (CT)a.fac1));
+}
+// ---------------------------------------------------------------------------
+// momentum + XSPH (+ integrator update)
+// ---------------------------------------------------------------------------
+template <typename CT>
+struct MomDst {
+  CT ha, rhoa, ca, Aa, alpha, beta, eta2;
+  float4 pd, vd;
+  int dim;
+};
+
+template <typename CT>
+__device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, const float4& vs,
+                                         const float4& as, bool do_m, bool do_x, CT* am,
+                                         CT* ax) {
+  CT rx, ry, rz;
+  const CT d2 = pair_d2<CT>(D.pd, ps, rx, ry, rz);
+  const CT hab = CT(0.5) * (D.ha + (CT)ps.w);
+  const CT inv = frcp<CT>(hab);
+  const CT rr = fsqrt<CT>(d2);
+  const CT qq = rr * inv;
+  const CT fac = h_dim_inv<CT>(inv, D.dim);
+  const CT mb = (CT)vs.w;
+  const CT rhob = (CT)as.x;
+  const CT irhoab = frcp<CT>(CT(0.5) * (D.rhoa + rhob));
+  if (do_m) {
+    const CT vx = (CT)D.vd.x - (CT)vs.x, vy = (CT)D.vd.y - (CT)vs.y, vz = (CT)D.vd.z - (CT)vs.z;
+    const CT dot = vx * rx + vy * ry + vz * rz;
+    const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
+    const CT cab = CT(0.5) * (D.ca + (CT)as.z);
+    const CT mu = hab * dot * frcp<CT>(d2 + D.eta2 * hab * hab);
+    const CT piab = (-D.alpha * cab * mu + D.beta * mu * mu) * irhoab;
+    const CT tmp = D.Aa + Ab + (dot < CT(0) ? piab : CT(0));
+    const CT gf = -mb * tmp * (g_sel<CT>(qq, rr, hab, inv) * fac);
+    am[0] += gf * rx;
+    am[1] += gf * ry;
+    am[2] += gf * rz;
+  }
+  if (do_x) {
+    const CT temp = mb * (w_sel<CT>(qq) * fac) * irhoab;
+    ax[0] += temp * ((CT)vs.x - (CT)D.vd.x);
+    ax[1] += temp * ((CT)vs.y - (CT)D.vd.y);
+    ax[2] += temp * ((CT)vs.z - (CT)D.vd.z);
+  }
+}
+
+template <typename CT>
+__device__ __forceinline__ MomDst<CT> mom_dst(const PassArgs<float>& a, int i, bool valid) {
+  const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
+  MomDst<CT> D;
+  D.pd = valid ? a.dst.pos4[i] : zero4;
+  D.vd = valid ? a.dst.vel4[i] : zero4;
+  const float4 ad = valid ? a.dst.aux4[i] : zero4;
+  D.dim = a.dim;
+  D.ha = (CT)D.pd.w;
+  D.rhoa = (CT)ad.x;
+  D.ca = (CT)ad.z;
+  D.Aa = sizeof(CT) == 8 ? (CT)ad.y / (D.rhoa * D.rhoa) : (CT)ad.w;
+  D.alpha = (CT)a.alpha;
+  D.beta = (CT)a.beta;
+  D.eta2 = (CT)(a.eta * a.eta);
+  return D;
+}
+
+// Outputs of the plain pass (MODE 0) or the integrator update of one
+// particle in the reference's per-calc order mom, gravity, step, xsph
+// (MODE 1 RK2 predictor, 2 RK2 final, 3 Euler); see momentum_kernel.
+template <typename CT, int MODE>
+__device__ __forceinline__ void mom_epilogue(const PassArgs<float>& a, int i, bool local,
+                                             CT* acc, CT* dv, const float4& vd) {
+  const CT f1 = (CT)a.fac1, fe = (CT)(a.fac1 * a.eps);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    acc[c] *= f1;
+    dv[c] *= fe;
+  }
+  if (MODE == 0) {
+    for (int c = 0; c < 3; ++c) {
+      if (a.do_mom && a.acc[c] != nullptr) a.acc[c][i] = (float)acc[c];
+      if (a.do_xsph && a.dv[c] != nullptr) a.dv[c][i] = (float)dv[c];
+    }
+    return;
+  }
+  const float dt = (float)a.dt;
+  const float hdt = (float)(0.5 * a.dt);
+  const float vel[3] = {vd.x, vd.y, vd.z};
+  float disp = 0.f;
+  for (int c = 0; c < a.dim; ++c) {
+    const float x0 = a.s0[c][i];
+    const float u0 = a.s0[3 + c][i];
+    const float km = local ? (float)acc[c] : 0.f;
+    const float kx = local ? (float)dv[c] : 0.f;
+    const float kg = (local && a.has_gravity) ? (float)a.g[c] : 0.f;
+    const float ks = local ? vel[c] : 0.f;
+    float xn, un;
+    if (MODE == 1 || MODE == 3) {
+      un = u0 + km * dt;
+      if (a.has_gravity) un = un + kg * dt;
+      xn = x0 + ks * dt;
+      if (a.do_xsph) xn = xn + kx * dt;
+      if (MODE == 1) {
+        a.k1[c][i] = km;
+        a.k1[3 + c][i] = kx;
+      }
+    } else {
+      const float k1m = a.k1[c][i], k1x = a.k1[3 + c][i];
+      un = u0 + hdt * (k1m + km);
+      if (a.has_gravity) un = un + hdt * (kg + kg);
+      xn = x0 + hdt * (u0 + ks);
+      if (a.do_xsph) xn = xn + hdt * (k1x + kx);
+    }
+    a.out[c][i] = xn;
+    a.out[3 + c][i] = un;
+    disp = fmaxf(disp, fabsf(xn - x0));
+  }
+  if (a.disp_out != nullptr) {
+    for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax((int*)a.disp_out, __float_as_int(disp));
+  }
+}
+
+// Cell scan with LDS queues (no list) for destination i; q as in
+// density_scan_body.
+template <typename CT, int MODE>
+__device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs<float>& a,
+                                                   int i, int* q) {
+  const bool valid = i < a.dst.n;
+  const bool local = valid && (a.dst.tag == nullptr || a.dst.tag[i] == 0);
+  const MomDst<CT> D = mom_dst<CT>(a, i, valid);
+  const double r = (double)D.pd.w * a.rs;
+  const double r2d = r * r;
+  const float r2f = (float)r2d;
+  int cx = 0, cy = 0, cz = 0;
+  if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
+  const int ring = ring_of(g, a.disp);
+  const Cull cu = make_cull(D.pd, r + (a.disp ? (double)*a.disp : 0.0), g);
+  CT acc[3] = {CT(0), CT(0), CT(0)};
+  CT dv[3] = {CT(0), CT(0), CT(0)};
+  for (int s = 0; s < a.nsrc; ++s) {
+    const View<float> S = a.src[s];
+    const bool do_m = (a.flags[s] & SPH_SRC_MOMENTUM) != 0 && a.do_mom;
+    const bool do_x = (a.flags[s] & SPH_SRC_XSPH) != 0 && a.do_xsph;
+    if (!do_m && !do_x) continue;
+    CT am[3] = {CT(0), CT(0), CT(0)};
+    CT ax[3] = {CT(0), CT(0), CT(0)};
+    int cnt = 0;
+    auto drain = [&]() {
+      const int m = wave_max(cnt);
+      for (int k = 0; k < m; ++k) {
+        if (k < cnt) {
+          const int j = q[k * 64];
+          mom_pair<CT>(D, S.pos4[j], S.vel4[j], S.aux4[j], do_m, do_x, am, ax);
+        }
+      }
+      cnt = 0;
+    };
+    for (int dz = -ring; dz <= ring; ++dz) {
+      for (int dy = -ring; dy <= ring; ++dy) {
+        int j0 = 0, j1 = 0;
+        if (local) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
+        while (__any(j0 < j1)) {
+          if (__any(cnt > kQueue - kUnroll)) drain();
+          float4 c[kUnroll];
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : kFar4;
+#pragma unroll
+          for (int u = 0; u < kUnroll; ++u) {
+            if (j0 + u < j1 && accept_exact(c[u], D.pd, r2f, r2d)) {
+              q[cnt * 64] = j0 + u;
+              ++cnt;
+            }
+          }
+          j0 += kUnroll;
+        }
+      }
+    }
+    drain();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      acc[c] += am[c];
+      dv[c] += ax[c];
+    }
+  }
+  if (!valid) return;
+  mom_epilogue<CT, MODE>(a, i, local, acc, dv, D.vd);
+}
+
+}  // namespace sph

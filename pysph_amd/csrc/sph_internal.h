@@ -31,6 +31,20 @@ int32_t check_view(const sph_view* a, const char* who, bool need_vel, bool need_
 int32_t check_grid(const sph_grid* g, const char* who);
 Grid make_grid(const sph_grid& g, double rmax);
 
+// Grid of one scene of a multi-scene binning.
+struct BatchBinGrid {
+  double cs;
+  int lo[3];
+  int dims[3];
+  int cell_base;
+};
+int32_t fill_i32(hipStream_t st, int32_t* p, int64_t n, int32_t v);
+int32_t batch_bin(hipStream_t st, int64_t n, int64_t ncells, const float* x, const float* y,
+                  const float* z, const int* scene, const BatchBinGrid* grids, int32_t* perm,
+                  int32_t* cell, int32_t* cell_start, void* ws);
+int32_t permute_fields(hipStream_t st, int64_t n, const int32_t* perm, int32_t nfields,
+                       const void* const* src, void* const* dst, const int32_t* esz);
+
 }  // namespace sph
 
 #define SPH_TRY(expr)                  \

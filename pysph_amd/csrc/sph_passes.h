@@ -79,6 +79,24 @@ struct PassArgs {
   float nl_skin;
 };
 
+// Scheme parameters of a pass from the ABI struct.
+template <typename T>
+inline void set_params(PassArgs<T>& a, const sph_params* prm) {
+  a.dim = prm->dim;
+  a.rs = prm->radius_scale;
+  a.fac1 = fac_unit(prm->dim);
+  a.co = prm->co;
+  a.ro = prm->ro;
+  a.gamma = prm->gamma;
+  a.B = prm->co * prm->co * prm->ro / prm->gamma;
+  a.alpha = prm->alpha;
+  a.beta = prm->beta;
+  a.eta = prm->eta;
+  a.eps = prm->eps;
+  for (int c = 0; c < 3; ++c) a.g[c] = prm->g[c];
+  a.has_gravity = prm->has_gravity;
+}
+
 // Tait equation of state in FP64 (eos_funcs.pyx:105-116).
 __device__ __forceinline__ void tait(double rho, double co, double ro, double gamma, double B,
                                      double& p, double& cs) {

@@ -262,22 +262,6 @@ static int32_t check_common(const sph_grid* g, const sph_params* prm, const sph_
   return SPH_OK;
 }
 
-template <typename T>
-static void set_params(PassArgs<T>& a, const sph_params* prm) {
-  a.dim = prm->dim;
-  a.rs = prm->radius_scale;
-  a.fac1 = fac_unit(prm->dim);
-  a.co = prm->co;
-  a.ro = prm->ro;
-  a.gamma = prm->gamma;
-  a.B = prm->co * prm->co * prm->ro / prm->gamma;
-  a.alpha = prm->alpha;
-  a.beta = prm->beta;
-  a.eta = prm->eta;
-  a.eps = prm->eps;
-  for (int c = 0; c < 3; ++c) a.g[c] = prm->g[c];
-  a.has_gravity = prm->has_gravity;
-}
 
 template <typename T, typename CT>
 static void launch_density(hipStream_t st, const Grid& G, const PassArgs<T>& a, bool tait) {

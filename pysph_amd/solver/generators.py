@@ -158,3 +158,57 @@ def cube_array(n, kind="uniform", seed=0, neighbours=50, precision="single"):
     return get_particle_array(cl_precision=precision, name="cube", type=ParticleType.Fluid,
                               x=x, y=y, z=z, m=np.full(n, 1.0 / n),
                               h=np.full(n, cube_h(n, neighbours)))
+
+
+def elliptical_drop_2d(n_target=5000, aspect=1.0, u0=100.0, hdx=1.3, rho0=1.0, alpha=0.1,
+                       c_ratio=14.0, precision="single"):
+    """2D elliptical drop of about n_target particles on a square lattice
+    inside the ellipse x^2 + (y/aspect)^2 <= 1, u = -u0 x, v = u0 y,
+    c0 = c_ratio * u0 (the reference's drop: c0 = 1400 at u0 = 100),
+    dt = 0.25 h / c0.  Returns (fluid, None, info)."""
+    a, b = 1.0, float(aspect)
+    dx = math.sqrt(math.pi * a * b / n_target)
+    R = int(math.ceil(max(a, b) / dx)) + 1
+    i, j = np.meshgrid(np.arange(-R, R + 1), np.arange(-R, R + 1), indexing="ij")
+    x, y = i.ravel() * dx, j.ravel() * dx
+    keep = (x / a) ** 2 + (y / b) ** 2 <= 1.0
+    x, y = x[keep], y[keep]
+    n = len(x)
+    dt_ = np.float64 if precision == "double" else np.float32
+    h = hdx * dx
+    co = c_ratio * u0
+    fluid = get_particle_array(
+        cl_precision=precision, name="fluid", type=ParticleType.Fluid,
+        x=_store(dt_, x), y=_store(dt_, y), z=np.zeros(n), u=-u0 * x, v=u0 * y,
+        w=np.zeros(n), m=np.full(n, rho0 * dx * dx), h=np.full(n, h), rho=np.full(n, rho0),
+        p=np.zeros(n), cs=np.zeros(n))
+    info = dict(dx=dx, h=h, rho0=rho0, co=co, gamma=7.0, alpha=alpha, beta=0.0, eta=0.1,
+                eps=0.0, g=None, dt=0.25 * h / co, n_fluid=n, n_boundary=0, u0=u0,
+                domain=((-1.2 * a, -1.6 * b, 0.0), (1.2 * a, 1.6 * b, 0.0)))
+    return fluid, None, info
+
+
+def batch_scenes(n_scenes=512, seed=0, n_min=1000, n_max=64000, precision="single"):
+    """Scenes of the batched benchmark: half elliptical drops, half 2D dam
+    breaks, fluid counts log-uniform in [n_min, n_max], per-scene h/dx in
+    [1.2, 1.5], drop aspect ratio in [0.5, 1], column height/width in
+    [0.5, 2] (width in [0.3, 1] m) and drop velocity scale u0 in [50, 150],
+    all drawn from `seed`.  Returns a list of (fluid, boundary, info)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(n_scenes):
+        n = int(round(math.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        hdx = float(rng.uniform(1.2, 1.5))
+        if k % 2 == 0:
+            f, b, info = elliptical_drop_2d(n, aspect=float(rng.uniform(0.5, 1.0)),
+                                            u0=float(rng.uniform(50.0, 150.0)), hdx=hdx,
+                                            precision=precision)
+        else:
+            width = float(rng.uniform(0.3, 1.0))
+            height = width * float(rng.uniform(0.5, 2.0))
+            dx = math.sqrt(width * height / n)
+            f, b, info = dam_break_2d(width=width, height=height, dx=dx, hdx=hdx,
+                                      precision=precision)
+        info["kind"] = "drop" if k % 2 == 0 else "dam"
+        out.append((f, b, info))
+    return out
