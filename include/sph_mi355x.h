@@ -352,6 +352,60 @@ int32_t sph_batch_density(void *handle, void *stream, int32_t phase, int32_t set
 int32_t sph_batch_stage(void *handle, void *stream, int32_t stage,
                         const sph_batch_sched *sched);
 
+/* ---- edge-MLP surrogate ---------------------------------------------------
+ * No counterpart in the reference.  A message-passing network over the
+ * symmetric radius graph of a particle set (every node a destination and a
+ * source, self edges included) predicts each fluid particle's acceleration
+ * (in units of g) over a time step 4x the plain one.  Node features:
+ * velocity / u_scale, (rho/rho0 - 1) rho_scale, Tait pressure, boundary
+ * flag.  Edge features: relative position / h, distance / h, relative
+ * velocity / u_scale, both densities, W, and the pair's pressure and
+ * artificial-viscosity accelerations (from the same cubic spline and Tait
+ * law).  Hidden width 128, 2 rounds; FP32 master weights with BF16 copies
+ * for the MFMA GEMMs, FP32 accumulation everywhere. */
+#define SPH_MLP_HIDDEN 128
+#define SPH_MLP_NIN 16
+#define SPH_MLP_NE 16
+#define SPH_MLP_ROUNDS 2
+typedef struct sph_mlp_graph {
+    int64_t n;            /* nodes */
+    int64_t nedges;
+    int64_t n_fluid;      /* nodes with btype 0 (loss normalisation) */
+    int32_t dim;          /* 2 or 3 */
+    int32_t reserved;
+    const float *pos4;    /* (x, y, z, h) per node */
+    const float *vel4;    /* (u, v, w, m) per node */
+    const float *rho;     /* density per node */
+    const int32_t *btype; /* 0 fluid, 1 boundary */
+    const int64_t *off;   /* CSR offsets, n + 1 */
+    const int64_t *nbr;   /* neighbour node of each edge */
+    double rho0, co, gamma, u_scale, g, rho_scale, alpha, eta;
+} sph_mlp_graph;
+int64_t sph_mlp_param_count(void);
+int32_t sph_mlp_create(void **model);
+int32_t sph_mlp_destroy(void *model);
+/* Parameters as FP32 in layout order (synchronous copies). */
+int32_t sph_mlp_set_params(void *model, const float *params_host);
+/* which 0: parameters, 1: accumulated gradients (synchronous). */
+int32_t sph_mlp_get(void *model, int32_t which, float *host);
+int32_t sph_mlp_zero_grad(void *model, void *stream);
+/* out_dev: n x 2 predicted accelerations / g.  Keeps the activations for a
+ * following sph_mlp_backward of the same graph. */
+int32_t sph_mlp_forward(void *model, void *stream, const sph_mlp_graph *g, float *out_dev);
+/* Mean over fluid nodes of |out - target|^2 -> *loss_dev; its gradient
+ * with respect to out -> dout_dev (n x 2). */
+int32_t sph_mlp_loss(void *model, void *stream, const sph_mlp_graph *g, const float *out_dev,
+                     const float *target_dev, float *loss_dev, float *dout_dev);
+/* Gradients of the parameters, added to the model's gradient buffer. */
+int32_t sph_mlp_backward(void *model, void *stream, const sph_mlp_graph *g,
+                         const float *dout_dev);
+int32_t sph_mlp_adam(void *model, void *stream, float lr, float beta1, float beta2, float eps,
+                     int32_t step);
+/* Surrogate step of the fluid nodes (btype 0): u += dt a_scale out,
+ * x += dt (u_old + u_new) / 2 (2D components). */
+int32_t sph_mlp_integrate(void *stream, int64_t n, const int32_t *btype, const float *out_dev,
+                          double a_scale, double dt, float *x, float *y, float *u, float *v);
+
 /* ---- stream capture into HIP graphs (any sequence of the calls above) -- */
 int32_t sph_graph_begin(void *stream);
 int32_t sph_graph_end(void *stream, void **graph_exec);

@@ -88,6 +88,20 @@ class SphBatchSched(ctypes.Structure):
                 ("grab", ctypes.c_int32), ("order", ctypes.c_int32)]
 
 
+SPH_MLP_HIDDEN = 128
+SPH_MLP_NIN = 16
+SPH_MLP_NE = 16
+SPH_MLP_ROUNDS = 2
+
+
+class SphMlpGraph(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int64), ("nedges", ctypes.c_int64), ("n_fluid", ctypes.c_int64),
+                ("dim", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+        (f, _vp) for f in ("pos4", "vel4", "rho", "btype", "off", "nbr")] + [
+        (f, ctypes.c_double) for f in ("rho0", "co", "gamma", "u_scale", "g", "rho_scale",
+                                       "alpha", "eta")]
+
+
 class SphError(RuntimeError):
     """A non-zero return code of the C ABI."""
 
@@ -147,6 +161,19 @@ _SIGS = {
     "sph_batch_density": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
                                            _P(SphBatchSched)]),
     "sph_batch_stage": (ctypes.c_int32, [_vp, _vp, ctypes.c_int32, _P(SphBatchSched)]),
+    "sph_mlp_param_count": (ctypes.c_int64, []),
+    "sph_mlp_create": (ctypes.c_int32, [_P(_vp)]),
+    "sph_mlp_destroy": (ctypes.c_int32, [_vp]),
+    "sph_mlp_set_params": (ctypes.c_int32, [_vp, _vp]),
+    "sph_mlp_get": (ctypes.c_int32, [_vp, ctypes.c_int32, _vp]),
+    "sph_mlp_zero_grad": (ctypes.c_int32, [_vp, _vp]),
+    "sph_mlp_forward": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp]),
+    "sph_mlp_loss": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp, _vp, _vp, _vp]),
+    "sph_mlp_backward": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp]),
+    "sph_mlp_adam": (ctypes.c_int32, [_vp, _vp, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_float, ctypes.c_int32]),
+    "sph_mlp_integrate": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, _vp, ctypes.c_double,
+                                           ctypes.c_double, _vp, _vp, _vp, _vp]),
     "sph_graph_begin": (ctypes.c_int32, [_vp]),
     "sph_graph_end": (ctypes.c_int32, [_vp, _P(_vp)]),
     "sph_graph_launch": (ctypes.c_int32, [_vp, _vp]),

@@ -12,7 +12,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libsph_mi355x.so")
 SOURCES = ["sph_abi.hip", "sph_bin.hip", "sph_passes.hip", "sph_fast.hip", "sph_integrate.hip",
-           "sph_batch.hip"]
+           "sph_batch.hip", "sph_mlp.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [

@@ -150,8 +150,11 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   const float w2 = cu.rc2 - gy * gy - gz * gz;
   if (!(w2 >= 0.f)) return;
   const float w = sqrtf(w2);
-  const float fa = floorf((cu.xd - w) * g.inv_cs_f) - (float)g.lo[0];
-  const float fb = floorf((cu.xd + w) * g.inv_cs_f) - (float)g.lo[0];
+  // cells of [x - w, x + w]; the border cells reach to infinity (particles
+  // outside the grid are clamped into them), so clamp the range into the grid
+  const float last = (float)(g.dims[0] - 1);
+  const float fa = fminf(fmaxf(floorf((cu.xd - w) * g.inv_cs_f) - (float)g.lo[0], 0.f), last);
+  const float fb = fminf(fmaxf(floorf((cu.xd + w) * g.inv_cs_f) - (float)g.lo[0], 0.f), last);
   int x0 = max(cx - ring, 0), x1 = min(cx + ring, g.dims[0] - 1);
   if (fa > (float)x1 || fb < (float)x0) return;  // NaN positions keep the ring
   if (fa > (float)x0) x0 = (int)fa;

@@ -1,0 +1,134 @@
+"""The GPU edge-MLP kernels (BF16 MFMA GEMMs, fused edge gather/scatter,
+forward and backward) against the FP64 network oracle on the same graph.
+
+Two references: the oracle with its matrix-product operands rounded to
+bfloat16 where the GPU stores them so (outputs and every gradient tensor
+within 1e-2 normwise relative: only the FP32 accumulation order differs),
+and the plain FP64 oracle (outputs within 3e-2, gradients within 0.2: the
+error that BF16 operands cost, reported)."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import mlp_oracle as M
+from pysph_amd.solver import generators as gen
+from pysph_amd.surrogate.graph import ParticleGraph, combined_array
+from pysph_amd.surrogate.model import EdgeMLP, init_params
+
+pytestmark = pytest.mark.gpu
+
+
+def _graph(width=0.2, height=0.3, dx=0.01):
+    f, b, info = gen.dam_break_2d(width=width, height=height, dx=dx, hdx=1.3,
+                                  precision="single", jitter=0.1, seed=4)
+    rng = np.random.Generator(np.random.PCG64(2))
+    nf = f.get_number_of_particles()
+    f.set(u=rng.normal(0, 0.5, nf), v=rng.normal(0, 0.5, nf))
+    g = ParticleGraph(combined_array(f, b), info, domain=info["domain"])
+    g.build()
+    return g, info
+
+
+def _host(g):
+    sc = g.pa.__dict__["_pack_scratch"]
+    n = g.n
+    pos4 = sc["pos4"][:n].double().cpu().numpy()
+    vel4 = sc["vel4"][:n].double().cpu().numpy()
+    rho = g.pa.get_carray("rho").double().cpu().numpy()
+    bt = g.pa.get_carray("btype").cpu().numpy()
+    off = g.off.cpu().numpy()
+    nbr = g.nbr[:g.nedges].cpu().numpy()
+    s = g.struct
+    c = dict(rho0=s.rho0, co=s.co, gamma=s.gamma, u_scale=s.u_scale, g=s.g,
+             rho_scale=s.rho_scale, alpha=s.alpha, eta=s.eta, dim=2)
+    return pos4, vel4, rho, bt, off, nbr, c
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) /
+                 max(np.linalg.norm(np.asarray(b)), 1e-30))
+
+
+def _params(seed):
+    theta = init_params(seed).astype(np.float64)
+    rng = np.random.Generator(np.random.PCG64(seed + 7))
+    p = M.unflatten(theta)
+    for k in p:
+        if p[k].ndim == 1:
+            p[k] = rng.normal(0, 0.1, p[k].shape)
+    return M.flatten(p).astype(np.float32)
+
+
+def test_graph_is_symmetric_with_self_edges():
+    g, _ = _graph()
+    off = g.off.cpu().numpy()
+    nbr = g.nbr[:g.nedges].cpu().numpy()
+    dst = np.repeat(np.arange(g.n), np.diff(off))
+    pairs = set(zip(dst.tolist(), nbr.tolist()))
+    assert all((b, a) in pairs for a, b in pairs)
+    assert all((k, k) in pairs for k in range(g.n))
+
+
+def test_forward_matches_oracle():
+    g, _ = _graph()
+    theta = _params(1)
+    model = EdgeMLP(params=theta)
+    y = model.forward(g)[:g.n].double().cpu().numpy()
+    pos4, vel4, rho, bt, off, nbr, c = _host(g)
+    X, E, dst, src = M.features(pos4, vel4, rho, bt, off, nbr, c)
+    p = M.unflatten(theta.astype(np.float64))
+    Y, _ = M.forward(p, X, E, dst, src, g.n)
+    Yb, _ = M.forward(p, X, E, dst, src, g.n, round_bf16=True)
+    err, errb = _rel(y, Y), _rel(y, Yb)
+    print("forward normwise rel err: %.3e vs FP64, %.3e vs BF16-rounded oracle" % (err, errb))
+    assert errb < 1e-2
+    assert err < 3e-2
+    assert np.abs(Y).max() > 1e-3
+
+
+def test_backward_matches_oracle():
+    g, _ = _graph()
+    theta = _params(2)
+    model = EdgeMLP(params=theta)
+    rng = np.random.Generator(np.random.PCG64(5))
+    T = rng.normal(0, 1, (g.n, 2))
+    tgt = torch.tensor(T, dtype=torch.float32, device=g.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    model.zero_grad(stream)
+    loss = float(model.loss_backward(g, tgt).item())
+    grads = model.grads()
+    pos4, vel4, rho, bt, off, nbr, c = _host(g)
+    X, E, dst, src = M.features(pos4, vel4, rho, bt, off, nbr, c)
+    p = M.unflatten(theta.astype(np.float64))
+    gg = M.unflatten(grads.astype(np.float64))
+    for round_bf16, tol in ((True, 1e-2), (False, 0.2)):
+        Y, cache = M.forward(p, X, E, dst, src, g.n, round_bf16=round_bf16)
+        L, dY = M.loss(Y, T, bt)
+        gref = M.backward(p, cache, dY)
+        assert abs(loss - L) < 1e-2 * L
+        worst = {}
+        for k in gref:
+            if k == "b_out":
+                a, b = gg[k][:2], gref[k][:2]
+            else:
+                a, b = gg[k], gref[k]
+            worst[k] = _rel(a, b)
+        print("gradient normwise rel errors (%s):" % ("BF16-rounded oracle" if round_bf16
+                                                      else "FP64 oracle"),
+              {k: "%.2e" % v for k, v in worst.items()})
+        assert max(worst.values()) < tol, worst
+
+
+def test_adam_steps_reduce_the_loss():
+    g, _ = _graph()
+    model = EdgeMLP(seed=3)
+    rng = np.random.Generator(np.random.PCG64(6))
+    tgt = torch.tensor(rng.normal(0, 1, (g.n, 2)) * 0.3, dtype=torch.float32, device=g.device)
+    stream = torch.cuda.current_stream().cuda_stream
+    losses = []
+    for _ in range(60):
+        model.zero_grad(stream)
+        losses.append(float(model.loss_backward(g, tgt).item()))
+        model.adam(stream, lr=3e-3)
+    print("loss %.3e -> %.3e" % (losses[0], losses[-1]))
+    assert losses[-1] < 0.7 * losses[0]

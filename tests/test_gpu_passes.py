@@ -169,3 +169,46 @@ def test_reference_four_particle_density_and_momentum():
     got = np.stack([pa.get("tmpx"), pa.get("tmpy"), pa.get("tmpz")])
     # reference test: assertAlmostEqual to 7 places
     assert np.allclose(got[:2], ref[:2], atol=1e-7, rtol=0)
+
+
+@pytest.mark.parametrize("cells_per_radius", [1, 2])
+def test_fast_path_particles_outside_fixed_domain(cells_per_radius):
+    """Particles outside a fixed binning domain are clamped into the border
+    cells; the culled stencil rows must still reach them (and they must
+    still find themselves)."""
+    rng = np.random.Generator(np.random.PCG64(11))
+    n = 4000
+    x, y = rng.random(n), rng.random(n)
+    pa = base.get_particle_array(cl_precision="single", name="cloud", x=x, y=y,
+                                 h=np.full(n, 0.02), m=np.full(n, 1.0 / n))
+    particles = base.Particles([pa], update_particles=False)
+    cm = particles.cell_manager
+    cm.set_fixed_domain((0.3, 0.3, 0.0), (0.7, 0.7, 0.0))
+    cm.set_cells_per_radius(cells_per_radius)
+    calc = sph.SPHCalc(particles, [pa], pa, base.CubicSplineKernel(dim=2),
+                       [sph.SPHRho.withargs().get_func(pa, pa)], updates=["rho"])
+    calc.sph("rho")
+    ref = to_oracle(pa)
+    rho_o = O.sphrho(2, ref, [ref])
+    assert max_rel(pa.get("rho"), rho_o) < 1e-5
+
+
+def test_fused_step_with_particles_outside_fixed_domain():
+    f, b, info = gen.dam_break_2d(width=0.2, height=0.3, dx=0.01, jitter=0.05, seed=5,
+                                  precision="single")
+    particles = base.Particles([f, b], update_particles=False)
+    cm = particles.cell_manager
+    # a domain that leaves the top of the column and the right wall outside
+    cm.set_fixed_domain((-0.03, -0.03, 0.0), (0.5, 0.2, 0.0))
+    cm.check_jumps = False
+    s = solver.wcsph_solver(2, co=info["co"], ro=info["rho0"], alpha=0.1, gravity=info["g"],
+                            xsph_eps=0.5)
+    s.setup_integrator(particles)
+    s.integrator.plan.set_cells_per_radius(2)
+    rf, rb = to_oracle(f), to_oracle(b)
+    s.integrator.integrate(info["dt"])
+    sch = O.scheme(2, integrator="rk2", dt=info["dt"], co=info["co"], ro=info["rho0"],
+                   gamma=7.0, alpha=0.1, beta=0.0, eta=0.1, gravity=info["g"], xsph_eps=0.5)
+    O.scheme_step(sch, [rf, rb], 1)
+    assert max_rel(f.get("rho"), rf["rho"]) < 1e-5
+    assert rel_norm(np.stack([f.get("u"), f.get("v")]), np.stack([rf["u"], rf["v"]])) < 1e-4
