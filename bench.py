@@ -25,11 +25,14 @@ def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dam3d",
-                    choices=["dam3d", "cube-uniform", "cube-clustered", "batch2d"],
+                    choices=["dam3d", "cube-uniform", "cube-clustered", "batch2d",
+                             "surrogate2d"],
                     help="dam3d: the 3D dam break step (default); cube-*: neighbour search + "
                          "summation density on 2^24 particles in the unit cube; batch2d: 512 "
                          "2D scenes (drops and dam breaks) stepped together")
     ap.add_argument("--scenes", type=int, default=512)
+    ap.add_argument("--surrogate-n", type=int, default=60000,
+                    help="surrogate2d: fluid particles of the 2D dam break")
     ap.add_argument("--no-graph", action="store_true", help="batch2d: launch eagerly")
     ap.add_argument("--no-autotune", action="store_true", help="batch2d: default schedule")
     ap.add_argument("--cube-n", type=int, default=1 << 24)
@@ -587,6 +590,101 @@ def batch_cpu_baseline(sc):
                       "single thread, %.1f s" % (used, total_t)}
 
 
+SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pysph_amd",
+                                 "surrogate", "weights", "dam2d_edge_mlp.npz")
+
+
+def main_surrogate(args):
+    """The trained edge-MLP surrogate (steps of 4 dt, checked against the
+    plain step every 10 steps) against the plain GPU solver on a 2D dam
+    break after 1000 plain steps of settling."""
+    import torch
+    from pysph_amd.surrogate.evaluate import compare
+    from pysph_amd.surrogate.graph import ParticleGraph, combined_array
+    from pysph_amd.surrogate.model import EdgeMLP
+    from pysph_amd.surrogate.train import dam_scene
+    model = EdgeMLP.load(SURROGATE_WEIGHTS)
+    untrained = EdgeMLP()
+    steps = args.steps if args.steps != 1000 else 100
+    r = compare(model, 7000, nsteps=steps, n=args.surrogate_n, warm=1000, baseline=untrained)
+    # forward-pass launch timing on a graph of the same scene
+    rng = np.random.Generator(np.random.PCG64(7000))
+    f, b, info = dam_scene(rng, (args.surrogate_n, args.surrogate_n))
+    g = ParticleGraph(combined_array(f, b), info, domain=info["domain"]).build()
+    model.forward(g)
+    model.timing(1)
+    for _ in range(10):
+        model.forward(g)
+    ms, calls = model.timing(0)
+    per = {k: v / calls for k, v in ms.items()}
+    n, ne = g.n, g.nedges
+    gemm_flops = 2.0 * n * (16 * 128 + 2 * (128 * 256 + 256 * 128))
+    # edge aggregation, per round: P, Q rows (2 x 512 B) and the BF16
+    # aggregate (256 B) per node, the 16-float feature row per edge
+    edge_bytes = 2 * (n * (1024 + 256) + ne * 64)
+    if per["gemm"] >= per["edges"]:
+        roof = {"bound": "mfma", "kernel": "sph::mlp::gemm_kernel (5 launches)",
+                "achieved": gemm_flops / (per["gemm"] * 1e-3) / 1e12, "peak": 2500.0,
+                "unit": "TFLOP/s", "flops_per_forward": gemm_flops, "avg_ms": per["gemm"]}
+    else:
+        roof = {"bound": "hbm", "kernel": "sph::mlp::edge_fwd_kernel (2 launches)",
+                "achieved": edge_bytes / (per["edges"] * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "bytes_per_forward": edge_bytes, "avg_ms": per["edges"]}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof["traffic"] = None
+    out = {
+        "metric": "simulated-time/s",
+        "value": r["surrogate_sim_time_per_s"],
+        "unit": "s/s",
+        "n_gpus": 1, "steps": steps, "warmup": 1000,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "bf16 network (FP32 accumulate), f32 particles", "data": "synthetic",
+        "config": {"workload": "2D dam break, edge-MLP surrogate at 4 dt vs plain RK2 at dt",
+                   "n_fluid": r["n_fluid"], "n_boundary": r["n_boundary"], "dt": r["dt"],
+                   "graph_nodes": n, "graph_edges": ne, "check_every": 10,
+                   "density_threshold": 0.01, "parallelism": "single GPU"},
+        "plain_sim_time_per_s": r["plain_sim_time_per_s"],
+        "speedup_vs_plain": r["surrogate_sim_time_per_s"] / r["plain_sim_time_per_s"],
+        "surrogate_density_error": r["surrogate_density_error"],
+        "surrogate_position_error_dx": r["surrogate_position_error"],
+        "fallbacks": r["fallbacks"], "checks": r["checks"],
+        "untrained_density_error": r.get("untrained_density_error"),
+        "plain_4dt_density_error": r["big_dt_density_error"],
+        "plain_4dt_position_error_dx": r["big_dt_position_error"],
+        "forward_ms": per,
+        "roofline": roof,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = surrogate_cpu_baseline(info)
+    print(json.dumps(out), flush=True)
+
+
+def surrogate_cpu_baseline(info):
+    """Plain RK2 steps of the FP64 oracle on the same 2D dam break, single
+    thread, until about 10 s of CPU work (simulated-time/s)."""
+    from oracle import oracle as O
+    from tests.util import to_oracle
+    from pysph_amd.surrogate.train import dam_scene
+    rng = np.random.Generator(np.random.PCG64(7000))
+    n = int(info.get("n_fluid", 60000))
+    f, b, inf = dam_scene(rng, (n, n))
+    arrs = [to_oracle(f, 0), to_oracle(b, 1)]
+    sch = O.scheme(2, integrator="rk2", dt=inf["dt"], co=inf["co"], ro=inf["rho0"], gamma=7.0,
+                   alpha=inf["alpha"], beta=inf["beta"], eta=inf["eta"], gravity=inf["g"],
+                   xsph_eps=inf["eps"], cell_size=2.0 * inf["h"])
+    O.set_threads(1)
+    steps, total = 0, 0.0
+    while total < 10.0 and steps < 50:
+        t0 = time.perf_counter()
+        O.scheme_step(sch, arrs, 1)
+        total += time.perf_counter() - t0
+        steps += 1
+    return {"value": steps * inf["dt"] / total, "unit": "s/s", "cores": 1, "kind": "port",
+            "sample": "%d RK2 steps of the same dam break (%d fluid + %d boundary particles) "
+                      "with the FP64 oracle, single thread, %.1f s"
+                      % (steps, f.get_number_of_particles(), b.get_number_of_particles(), total)}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -604,6 +702,8 @@ def main():
         return
     if args.workload == "batch2d":
         return main_batch(args)
+    if args.workload == "surrogate2d":
+        return main_surrogate(args)
     if args.workload != "dam3d":
         return main_cube(args)
     from pysph_amd.sph import _ops

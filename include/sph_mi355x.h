@@ -249,6 +249,9 @@ int32_t sph_wcsph_stage(void *stream, int32_t stage, const sph_grid *g,
  * mode 3: out = a                        (set_initial_arrays / reset copy,
  *                                         PositionStepping k = u, :40-60)
  * mode 4: out = s                        (GravityForce k = g, :39-45)
+ * mode 5: out = (a + 2b) + 2c            (RK4 partial sum k1 + 2k2 + 2k3, :669-689)
+ * mode 7: out = 2a - b                    (PredictorCorrector final_step, :756-772)
+ * mode 8: out = a + s*(b - c)            (LeapFrog final_step, s = 0.5*dt, :866-883)
  * With tag_dev non-NULL, modes 3 and 4 write 0 where tag != LocalReal
  * (sph_func.pyx:211-234).  nops arrays of n elements each, one dtype. */
 typedef struct sph_update_op {
@@ -259,6 +262,12 @@ typedef struct sph_update_op {
 } sph_update_op;
 int32_t sph_update(void *stream, int32_t dtype, int64_t n, int32_t mode, double s,
                    int32_t nops, const sph_update_op *ops_host, const int64_t *tag_dev);
+
+/* MoveCircleX (axis 0) / MoveCircleY (axis 1) of external_force.pyx:99-162:
+ * the components of unit-speed circular motion about the origin into
+ * out0_dev / out1_dev (either may be NULL).  Reads x, y, z (and tag). */
+int32_t sph_move_circle(void *stream, const sph_view *a, int32_t axis, void *out0_dev,
+                        void *out1_dev);
 
 /* FP64 conservation sums over LocalReal particles of one array, added to
  * out_dev[8]: mass, momentum x,y,z, kinetic energy, -sum m g.x, Tait elastic
@@ -389,6 +398,12 @@ int32_t sph_mlp_set_params(void *model, const float *params_host);
 /* which 0: parameters, 1: accumulated gradients (synchronous). */
 int32_t sph_mlp_get(void *model, int32_t which, float *host);
 int32_t sph_mlp_zero_grad(void *model, void *stream);
+/* Launch timing of sph_mlp_forward (synchronises after each timed call):
+ * ms_host[4] receives the accumulated milliseconds of the feature, GEMM,
+ * edge-aggregation and decoder launches and *calls_host the number of timed
+ * calls (either may be NULL); enable 1 / 0 switches timing on / off and
+ * resets the totals, -1 only reads them. */
+int32_t sph_mlp_timing(void *model, int32_t enable, double *ms_host, int64_t *calls_host);
 /* out_dev: n x 2 predicted accelerations / g.  Keeps the activations for a
  * following sph_mlp_backward of the same graph. */
 int32_t sph_mlp_forward(void *model, void *stream, const sph_mlp_graph *g, float *out_dev);
@@ -401,8 +416,8 @@ int32_t sph_mlp_backward(void *model, void *stream, const sph_mlp_graph *g,
                          const float *dout_dev);
 int32_t sph_mlp_adam(void *model, void *stream, float lr, float beta1, float beta2, float eps,
                      int32_t step);
-/* Surrogate step of the fluid nodes (btype 0): u += dt a_scale out,
- * x += dt (u_old + u_new) / 2 (2D components). */
+/* Surrogate step of the fluid nodes (btype 0), symplectic Euler:
+ * u += dt a_scale out, then x += dt u (2D components). */
 int32_t sph_mlp_integrate(void *stream, int64_t n, const int32_t *btype, const float *out_dev,
                           double a_scale, double dt, float *x, float *y, float *u, float *v);
 

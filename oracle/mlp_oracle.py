@@ -124,11 +124,14 @@ def features(pos4, vel4, rho, btype, off, nbr, c):
     mu = h * dot / (r * r + c["eta"] ** 2 * h * h)
     visc = np.where(dot < 0, -mbar * (-c["alpha"] * cbar * mu) / (0.5 * (ri + rj)) * gf / c["g"],
                     0.0)
-    # node sums of the pair pressure / viscosity accelerations (units of g)
-    np.add.at(X[:, 6], dst, pt * d[:, 0])
-    np.add.at(X[:, 7], dst, pt * d[:, 1])
-    np.add.at(X[:, 8], dst, visc * d[:, 0])
-    np.add.at(X[:, 9], dst, visc * d[:, 1])
+    # node sums of the pair pressure / viscosity accelerations (units of g);
+    # the network reads them scaled by 0.01, the decoder's skip path unscaled
+    acc = np.zeros((n, 4))
+    np.add.at(acc[:, 0], dst, pt * d[:, 0])
+    np.add.at(acc[:, 1], dst, pt * d[:, 1])
+    np.add.at(acc[:, 2], dst, visc * d[:, 0])
+    np.add.at(acc[:, 3], dst, visc * d[:, 1])
+    X[:, 6:10] = 0.01 * acc
     ps = 2.0 * c["g"] * h / c["u_scale"] ** 2
     E = np.zeros((len(src), NE))
     E[:, 0] = d[:, 0] / h
@@ -181,7 +184,7 @@ def forward(p, X, E, dst, src, n, round_bf16=False):
         Hc = Hc + U
     cache["H_last"] = Hc
     # skip path: the node's SPH pressure + viscosity acceleration / g
-    Y = Hc @ p["w_out"] + p["b_out"][:NOUT] + X[:, 6:8] + X[:, 8:10]
+    Y = Hc @ p["w_out"] + p["b_out"][:NOUT] + 100.0 * (X[:, 6:8] + X[:, 8:10])
     return Y, cache
 
 

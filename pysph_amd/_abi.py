@@ -150,6 +150,7 @@ _SIGS = {
     "sph_update": (ctypes.c_int32, [_vp, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                     ctypes.c_double, ctypes.c_int32, _P(SphUpdateOp), _vp]),
     "sph_conservation_f64": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp]),
+    "sph_move_circle": (ctypes.c_int32, [_vp, _P(SphView), ctypes.c_int32, _vp, _vp]),
     "sph_fill_i32": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32]),
     "sph_pack": (ctypes.c_int32, [_vp, _P(SphView), _vp, _vp]),
     "sph_batch_ncells": (ctypes.c_int64, [ctypes.c_int32, _P(SphBatchScene)]),
@@ -167,6 +168,7 @@ _SIGS = {
     "sph_mlp_set_params": (ctypes.c_int32, [_vp, _vp]),
     "sph_mlp_get": (ctypes.c_int32, [_vp, ctypes.c_int32, _vp]),
     "sph_mlp_zero_grad": (ctypes.c_int32, [_vp, _vp]),
+    "sph_mlp_timing": (ctypes.c_int32, [_vp, ctypes.c_int32, _vp, _vp]),
     "sph_mlp_forward": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp]),
     "sph_mlp_loss": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp, _vp, _vp, _vp]),
     "sph_mlp_backward": (ctypes.c_int32, [_vp, _vp, _P(SphMlpGraph), _vp]),

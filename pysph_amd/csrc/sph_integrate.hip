@@ -35,6 +35,12 @@ __global__ __launch_bounds__(kBlock) void update_kernel(UpdArgs<T> u) {
       r = u.a[k][i] + u.b[k][i] * u.s;
     } else if (MODE == 1) {
       r = u.a[k][i] + u.s * (u.b[k][i] + u.c[k][i]);
+    } else if (MODE == 5) {
+      r = (u.a[k][i] + T(2) * u.b[k][i]) + T(2) * u.c[k][i];
+    } else if (MODE == 7) {
+      r = T(2) * u.a[k][i] - u.b[k][i];
+    } else if (MODE == 8) {
+      r = u.a[k][i] + u.s * (u.b[k][i] - u.c[k][i]);
     } else if (MODE == 3) {
       r = (u.tag == nullptr || u.tag[i] == 0) ? u.a[k][i] : T(0);
     } else {
@@ -56,8 +62,8 @@ static int32_t update_impl(hipStream_t st, int64_t n, int mode, double s, int no
     u.tag = tag;
     for (int k = 0; k < u.nops; ++k) {
       const sph_update_op& op = ops[k0 + k];
-      if (!op.out || (mode != 4 && !op.a) || ((mode == 0 || mode == 1 || mode == 2) && !op.b) ||
-          (mode == 1 && !op.c)) {
+      if (!op.out || (mode != 4 && !op.a) || (mode != 3 && mode != 4 && !op.b) ||
+          ((mode == 1 || mode == 5 || mode == 8) && !op.c)) {
         set_error("sph_update: op %d has a NULL operand", k0 + k);
         return SPH_EINVAL;
       }
@@ -81,12 +87,48 @@ static int32_t update_impl(hipStream_t st, int64_t n, int mode, double s, int no
       case 3:
         hipLaunchKernelGGL((update_kernel<T, 3>), dim3(nb), dim3(kBlock), 0, st, u);
         break;
+      case 5:
+        hipLaunchKernelGGL((update_kernel<T, 5>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      case 7:
+        hipLaunchKernelGGL((update_kernel<T, 7>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
+      case 8:
+        hipLaunchKernelGGL((update_kernel<T, 8>), dim3(nb), dim3(kBlock), 0, st, u);
+        break;
       default:
         hipLaunchKernelGGL((update_kernel<T, 4>), dim3(nb), dim3(kBlock), 0, st, u);
         break;
     }
   }
   return launch_status("sph_update");
+}
+
+// ---------------------------------------------------------------------------
+// MoveCircleX / MoveCircleY (external_force.pyx:99-162): the velocity of
+// unit-speed motion on a circle about the origin, from the angle
+// arccos(x / |p|).  Non-LocalReal particles get 0 (sph_func.pyx:211-234).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void circle_kernel(View<T> a, int axis,
+                                                        T* __restrict__ o0,
+                                                        T* __restrict__ o1) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.n) return;
+  double r0 = 0.0, r1 = 0.0;
+  if (a.tag == nullptr || a.tag[i] == 0) {
+    const double x = (double)a.x[i], y = (double)a.y[i], z = (double)a.z[i];
+    const double len = sqrt(x * x + y * y + z * z);
+    const double angle = acos(x / len);
+    if (axis == 0) {
+      r0 = -sin(angle);
+      if (y < 0) r0 *= -1.0;
+    } else {
+      r1 = cos(angle);
+    }
+  }
+  if (o0) o0[i] = (T)r0;
+  if (o1) o1[i] = (T)r1;
 }
 
 // ---------------------------------------------------------------------------
@@ -145,7 +187,7 @@ extern "C" {
 
 int32_t sph_update(void* stream, int32_t dtype, int64_t n, int32_t mode, double s, int32_t nops,
                    const sph_update_op* ops_host, const int64_t* tag_dev) {
-  if (n < 0 || nops < 0 || (nops > 0 && !ops_host) || mode < 0 || mode > 4) {
+  if (n < 0 || nops < 0 || (nops > 0 && !ops_host) || mode < 0 || mode > 8 || mode == 6) {
     set_error("sph_update: bad arguments");
     return SPH_EINVAL;
   }
@@ -153,6 +195,25 @@ int32_t sph_update(void* stream, int32_t dtype, int64_t n, int32_t mode, double 
   if (dtype == SPH_F64) return update_impl<double>(as_stream(stream), n, mode, s, nops, ops_host, tag_dev);
   set_error("sph_update: dtype unsupported");
   return SPH_EUNSUPPORTED;
+}
+
+int32_t sph_move_circle(void* stream, const sph_view* a, int32_t axis, void* out0_dev,
+                        void* out1_dev) {
+  SPH_TRY(check_view(a, "sph_move_circle", false, false, false));
+  if (axis < 0 || axis > 1) {
+    set_error("sph_move_circle: axis must be 0 (x) or 1 (y)");
+    return SPH_EINVAL;
+  }
+  if (a->n == 0) return SPH_OK;
+  hipStream_t st = as_stream(stream);
+  const int nb = nblocks_for(a->n);
+  if (a->dtype == SPH_F32)
+    hipLaunchKernelGGL(circle_kernel<float>, dim3(nb), dim3(kBlock), 0, st, to_view<float>(*a),
+                       (int)axis, (float*)out0_dev, (float*)out1_dev);
+  else
+    hipLaunchKernelGGL(circle_kernel<double>, dim3(nb), dim3(kBlock), 0, st,
+                       to_view<double>(*a), (int)axis, (double*)out0_dev, (double*)out1_dev);
+  return launch_status("sph_move_circle");
 }
 
 int32_t sph_conservation_f64(void* stream, const sph_params* prm, const sph_view* a,

@@ -127,6 +127,67 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   }
 }
 
+// The same product for op(A) = A, op(B) = B (the forward layers and the
+// input gradients): each lane loads its 8 k-values of A as one 16-byte load,
+// and the block stages its 64-column panel of B in LDS transposed (column
+// major, k contiguous) so that every B fragment is one 16-byte LDS read.
+// Requires lda % 8 == 0, K % 16 == 0 and K <= kMaxK.
+constexpr int kMaxK = 256;
+constexpr int kLdsK = kMaxK + 8;  // padded pitch (bf16 elements)
+
+__device__ __forceinline__ void epilogue(const GemmArgs& g, const f32x16& acc, int orow0,
+                                         int col, int lane, float* C) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int orow = orow0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+    if (orow >= g.M || col >= g.N) continue;
+    float v = acc[r];
+    if (g.bias) v += g.bias[col];
+    if (g.relu) v = fmaxf(v, 0.f);
+    if (g.U) g.U[(int64_t)orow * g.ldu + col] = v;
+    if (g.res) v += g.res[(int64_t)orow * g.ldr + col];
+    if (C) {
+      float* p = C + (int64_t)orow * g.ldc + col;
+      if (g.accumulate) v += *p;
+      *p = v;
+    }
+    if (g.Cb) g.Cb[(int64_t)orow * g.ldcb + col] = (__bf16)v;
+  }
+}
+
+// C = A B^T-stored: A [M x K] row-major, B given as B^T [N x K] row-major
+// (k contiguous for both), so every fragment is one 16-byte load from
+// global memory / L2 and no LDS staging is needed.  The forward layers use
+// transposed BF16 copies of the weights; the input gradients use the
+// weights themselves.  Requires lda, ldb % 8 == 0 and K % 16 == 0.
+__global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int wm = w & 1, wn = w >> 1;
+  const int row = blockIdx.y * 64 + wm * 32 + (lane & 31);
+  const int col = blockIdx.x * 64 + wn * 32 + (lane & 31);
+  const int kh = 8 * (lane >> 5);
+  const bool rok = row < g.M, cok = col < g.N;
+  const __bf16* ap = g.A + (int64_t)(rok ? row : 0) * g.lda + kh;
+  const __bf16* bp = g.B + (int64_t)(cok ? col : 0) * g.ldb + kh;
+  f32x16 acc = {};
+  for (int k0 = 0; k0 < g.K; k0 += 16) {
+    const bf16x8 a = rok ? *(const bf16x8*)(ap + k0) : bf16x8{};
+    const bf16x8 b = cok ? *(const bf16x8*)(bp + k0) : bf16x8{};
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+  epilogue(g, acc, blockIdx.y * 64 + wm * 32, col, lane, g.C);
+}
+
+// wt[n*K + k] = bf16(w[k*N + n]) for one K x N weight matrix.
+__global__ __launch_bounds__(256) void transpose_bf16_kernel(int K, int N,
+                                                             const float* __restrict__ w,
+                                                             __bf16* __restrict__ wt) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  if (t >= K * N) return;
+  const int k = t / N, n = t % N;
+  wt[(int64_t)n * K + k] = (__bf16)w[t];
+}
+
 // Sum of nslab slabs [M x N] (ld) into out (accumulate when acc != 0).
 __global__ __launch_bounds__(256) void slab_sum_kernel(const float* __restrict__ part, int nslab,
                                                        int64_t zstride, int M, int N,
@@ -195,8 +256,11 @@ __global__ __launch_bounds__(256) void node_feat_kernel(FeatArgs f, const float*
   x[3] = tait_p(f, rho) / f.B;
   x[4] = f.btype[i] ? 1.f : 0.f;
   x[5] = 1.f;
+  // the SPH accelerations enter the network scaled down (they reach 100s of
+  // g at a summation-density free surface); the decoder's skip path adds
+  // them unscaled
 #pragma unroll
-  for (int c = 0; c < 4; ++c) x[6 + c] = acc4[(int64_t)i * 4 + c];
+  for (int c = 0; c < 4; ++c) x[6 + c] = 0.01f * acc4[(int64_t)i * 4 + c];
 #pragma unroll
   for (int k = 0; k < kIn; ++k) X[(int64_t)i * kIn + k] = (__bf16)x[k];
 }
@@ -311,19 +375,31 @@ __global__ __launch_bounds__(256) void edge_fwd_kernel(EdgeArgs a) {
   const float p1 = a.PQ[(int64_t)i * 2 * kH + lane + 64] + a.be[lane + 64];
   float s0 = 0.f, s1 = 0.f;
   const int64_t e0 = a.off[i], e1 = a.off[i + 1];
-  for (int64_t e = e0; e < e1; ++e) {
-    const int j = (int)a.nbr[e];
-    const float* x = a.E + e * kE;
-    float z0 = p0 + a.PQ[(int64_t)j * 2 * kH + kH + lane];
-    float z1 = p1 + a.PQ[(int64_t)j * 2 * kH + kH + lane + 64];
+  // four edges per iteration: their neighbour rows are in flight together
+  constexpr int U = 4;
+  for (int64_t e = e0; e < e1; e += U) {
+    float q0[U], q1[U];
 #pragma unroll
-    for (int k = 0; k < kE; ++k) {
-      const float xk = x[k];
-      z0 += xk * wc0[k];
-      z1 += xk * wc1[k];
+    for (int u = 0; u < U; ++u) {
+      const int64_t eu = e + u < e1 ? e + u : e0;
+      const int j = (int)a.nbr[eu];
+      q0[u] = a.PQ[(int64_t)j * 2 * kH + kH + lane];
+      q1[u] = a.PQ[(int64_t)j * 2 * kH + kH + lane + 64];
     }
-    s0 += fmaxf(z0, 0.f);
-    s1 += fmaxf(z1, 0.f);
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (e + u >= e1) break;
+      const float* x = a.E + (e + u) * kE;
+      float z0 = p0 + q0[u], z1 = p1 + q1[u];
+#pragma unroll
+      for (int k = 0; k < kE; ++k) {
+        const float xk = x[k];
+        z0 += xk * wc0[k];
+        z1 += xk * wc1[k];
+      }
+      s0 += fmaxf(z0, 0.f);
+      s1 += fmaxf(z1, 0.f);
+    }
   }
   a.A[(int64_t)i * a.lda + lane] = (__bf16)s0;
   a.A[(int64_t)i * a.lda + lane + 64] = (__bf16)s1;
@@ -574,7 +650,7 @@ __global__ __launch_bounds__(256) void fill_f32_kernel(int64_t n, float* __restr
   if (i < n) p[i] = v;
 }
 
-// Surrogate update of fluid nodes: u += dt a g, x += dt (u_old + u_new) / 2.
+// Surrogate update of fluid nodes (symplectic Euler): u += dt a g, x += dt u.
 __global__ __launch_bounds__(256) void integrate_kernel(int64_t n, const int* __restrict__ btype,
                                                         const float* __restrict__ Y, float ag,
                                                         float dt, float* __restrict__ x,
@@ -583,13 +659,12 @@ __global__ __launch_bounds__(256) void integrate_kernel(int64_t n, const int* __
                                                         float* __restrict__ v) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i >= n || btype[i] != 0) return;
-  const float u0 = u[i], v0 = v[i];
-  const float u1 = u0 + dt * ag * Y[2 * i];
-  const float v1 = v0 + dt * ag * Y[2 * i + 1];
+  const float u1 = u[i] + dt * ag * Y[2 * i];
+  const float v1 = v[i] + dt * ag * Y[2 * i + 1];
   u[i] = u1;
   v[i] = v1;
-  x[i] = x[i] + dt * 0.5f * (u0 + u1);
-  y[i] = y[i] + dt * 0.5f * (v0 + v1);
+  x[i] = x[i] + dt * u1;
+  y[i] = y[i] + dt * v1;
 }
 
 // ---------------------------------------------------------------------------
@@ -604,12 +679,56 @@ struct Model {
   Layout L;
   float* w = nullptr;     // FP32 master
   __bf16* wb = nullptr;   // BF16 copy
+  __bf16* wbt = nullptr;  // BF16 copies of the GEMM weights, transposed
   float* g = nullptr;     // gradients
   float* m = nullptr;     // Adam moments
   float* v = nullptr;
   // activations and scratch, grown on demand
+  // optional per-category launch timing of the forward pass: features,
+  // GEMMs, edge aggregation, decoder
+  int timing = 0;
+  double t_ms[4] = {0, 0, 0, 0};
+  int64_t t_calls = 0;
+  std::vector<hipEvent_t> ev;
   Buf X, E, N4, H[kR + 1], HA[kR], PQ[kR], U[kR], dH, dZ, dZb, dHA, dPQ, dPQb, part, lossp;
   int64_t n_act = -1, e_act = -1;
+};
+
+// Events around each launch of a timed forward pass.
+struct Timer {
+  Model* M;
+  hipStream_t st;
+  std::vector<std::pair<int, int>> marks;  // (category, event index of the start)
+  int used = 0;
+  Timer(Model* m, hipStream_t s) : M(m), st(s) {}
+  void begin(int cat) {
+    if (!M->timing) return;
+    while ((int)M->ev.size() < used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) {
+        M->timing = 0;
+        return;
+      }
+      M->ev.push_back(e);
+    }
+    (void)hipEventRecord(M->ev[used], st);
+    marks.push_back({cat, used});
+    used += 2;
+  }
+  void end() {
+    if (!M->timing || marks.empty()) return;
+    (void)hipEventRecord(M->ev[marks.back().second + 1], st);
+  }
+  void collect() {
+    if (!M->timing || marks.empty()) return;
+    (void)hipEventSynchronize(M->ev[marks.back().second + 1]);
+    for (auto& m : marks) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, M->ev[m.second], M->ev[m.second + 1]) == hipSuccess)
+        M->t_ms[m.first] += ms;
+    }
+    M->t_calls += 1;
+  }
 };
 
 static int32_t ensure(Buf& b, size_t bytes) {
@@ -633,6 +752,13 @@ static int32_t ensure(Buf& b, size_t bytes) {
 static int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
 static int32_t gemm(hipStream_t st, bool ta, bool tb, GemmArgs a, int splits = 1) {
+  if (!ta && tb && splits == 1 && a.K % 16 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0 &&
+      ((uintptr_t)a.A & 15) == 0 && ((uintptr_t)a.B & 15) == 0) {
+    a.kc = a.K;
+    dim3 grid((a.N + 63) / 64, (a.M + 63) / 64, 1);
+    hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(256), 0, st, a);
+    return launch_status("sph_mlp/gemm_nt");
+  }
   a.kc = (a.K + splits - 1) / splits;
   a.kc = (a.kc + 15) / 16 * 16;
   const int z = (a.K + a.kc - 1) / a.kc;
@@ -722,6 +848,24 @@ static int32_t check_graph(const sph_mlp_graph* g, const char* who) {
   return SPH_OK;
 }
 
+// Transposed BF16 copies of the GEMM weights (W_in, W_ab, W_u).
+static int32_t refresh_transposed(Model* M, hipStream_t st) {
+  const Layout& L = M->L;
+  struct Mat {
+    int64_t off;
+    int K, N;
+  };
+  std::vector<Mat> mats = {{L.w_in, kIn, kH}};
+  for (int r = 0; r < kR; ++r) {
+    mats.push_back({L.w_ab[r], kH, 2 * kH});
+    mats.push_back({L.w_u[r], 2 * kH, kH});
+  }
+  for (const Mat& m : mats)
+    hipLaunchKernelGGL(transpose_bf16_kernel, dim3(nblk((int64_t)m.K * m.N)), dim3(256), 0, st,
+                       m.K, m.N, M->w + m.off, M->wbt + m.off);
+  return launch_status("sph_mlp/transpose");
+}
+
 }  // namespace mlp
 }  // namespace sph
 
@@ -742,6 +886,7 @@ int32_t sph_mlp_create(void** model) {
   const size_t n = (size_t)M->L.total;
   hipError_t e = hipMalloc((void**)&M->w, n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&M->wb, n * 2);
+  if (e == hipSuccess) e = hipMalloc((void**)&M->wbt, n * 2);
   if (e == hipSuccess) e = hipMalloc((void**)&M->g, n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&M->m, n * 4);
   if (e == hipSuccess) e = hipMalloc((void**)&M->v, n * 4);
@@ -761,8 +906,10 @@ int32_t sph_mlp_create(void** model) {
 int32_t sph_mlp_destroy(void* model) {
   Model* M = (Model*)model;
   if (!M) return SPH_OK;
-  for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->g, (void*)M->m, (void*)M->v})
+  for (void* p : {(void*)M->w, (void*)M->wb, (void*)M->wbt, (void*)M->g, (void*)M->m,
+                  (void*)M->v})
     (void)hipFree(p);
+  for (hipEvent_t e : M->ev) (void)hipEventDestroy(e);
   for (Buf* b : {&M->X, &M->E, &M->N4, &M->dH, &M->dZ, &M->dZb, &M->dHA, &M->dPQ, &M->dPQb, &M->part,
                  &M->lossp})
     (void)hipFree(b->p);
@@ -790,6 +937,7 @@ int32_t sph_mlp_set_params(void* model, const float* host) {
   }
   hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(M->L.total)), dim3(256), 0, 0, M->L.total, M->w,
                      M->wb);
+  SPH_TRY(refresh_transposed(M, 0));
   e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     set_error("sph_mlp_set_params: %s", hipGetErrorString(e));
@@ -810,6 +958,23 @@ int32_t sph_mlp_get(void* model, int32_t which, float* host) {
   if (e != hipSuccess) {
     set_error("sph_mlp_get: %s", hipGetErrorString(e));
     return (int32_t)e;
+  }
+  return SPH_OK;
+}
+
+int32_t sph_mlp_timing(void* model, int32_t enable, double* ms_host, int64_t* calls_host) {
+  Model* M = (Model*)model;
+  if (!M) {
+    set_error("sph_mlp_timing: NULL model");
+    return SPH_EINVAL;
+  }
+  if (ms_host)
+    for (int k = 0; k < 4; ++k) ms_host[k] = M->t_ms[k];
+  if (calls_host) *calls_host = M->t_calls;
+  if (enable >= 0) {
+    M->timing = enable ? 1 : 0;
+    for (int k = 0; k < 4; ++k) M->t_ms[k] = 0.0;
+    M->t_calls = 0;
   }
   return SPH_OK;
 }
@@ -847,31 +1012,38 @@ int32_t sph_mlp_forward(void* model, void* stream, const sph_mlp_graph* gr, floa
   M->n_act = n;
   M->e_act = ne;
   SPH_TRY(ensure(M->N4, (size_t)n * 4 * 4));
+  Timer T(M, st);
   const FeatArgs f = feat_args(gr);
+  T.begin(0);
   hipLaunchKernelGGL(edge_feat_kernel, dim3((int)((n + 3) / 4)), dim3(256), 0, st, f,
                      (float*)M->E.p, (float*)M->N4.p);
   hipLaunchKernelGGL(node_feat_kernel, dim3(nblk(n)), dim3(256), 0, st, f,
                      (const float*)M->N4.p, (__bf16*)M->X.p);
+  T.end();
   SPH_TRY(launch_status("sph_mlp_forward/features"));
   // encoder: H0 = ReLU(X W_in + b_in) -> H[0] (FP32), HA[0] left half (BF16)
   {
-    GemmArgs a = gargs((int)n, kH, kIn, (const __bf16*)M->X.p, kIn, M->wb + L.w_in, kH);
+    GemmArgs a = gargs((int)n, kH, kIn, (const __bf16*)M->X.p, kIn, M->wbt + L.w_in, kIn);
     a.C = (float*)M->H[0].p;
     a.ldc = kH;
     a.bias = M->w + L.b_in;
     a.relu = 1;
     a.Cb = (__bf16*)M->HA[0].p;
     a.ldcb = 2 * kH;
-    SPH_TRY(gemm(st, false, false, a));
+    T.begin(1);
+    SPH_TRY(gemm(st, false, true, a));
+    T.end();
   }
   for (int r = 0; r < kR; ++r) {
     __bf16* HA = (__bf16*)M->HA[r].p;
     float* PQ = (float*)M->PQ[r].p;
     {  // [P | Q] = H W_ab
-      GemmArgs a = gargs((int)n, 2 * kH, kH, HA, 2 * kH, M->wb + L.w_ab[r], 2 * kH);
+      GemmArgs a = gargs((int)n, 2 * kH, kH, HA, 2 * kH, M->wbt + L.w_ab[r], kH);
       a.C = PQ;
       a.ldc = 2 * kH;
-      SPH_TRY(gemm(st, false, false, a));
+      T.begin(1);
+      SPH_TRY(gemm(st, false, true, a));
+      T.end();
     }
     EdgeArgs e{};
     e.n = (int)n;
@@ -883,10 +1055,12 @@ int32_t sph_mlp_forward(void* model, void* stream, const sph_mlp_graph* gr, floa
     e.be = M->w + L.b_e[r];
     e.A = HA + kH;
     e.lda = 2 * kH;
+    T.begin(2);
     hipLaunchKernelGGL(edge_fwd_kernel, dim3((int)((n + 3) / 4)), dim3(256), 0, st, e);
+    T.end();
     SPH_TRY(launch_status("sph_mlp_forward/edges"));
     {  // U = ReLU([H | A] W_u + c_u); H' = H + U
-      GemmArgs a = gargs((int)n, kH, 2 * kH, HA, 2 * kH, M->wb + L.w_u[r], kH);
+      GemmArgs a = gargs((int)n, kH, 2 * kH, HA, 2 * kH, M->wbt + L.w_u[r], 2 * kH);
       a.bias = M->w + L.c_u[r];
       a.relu = 1;
       a.U = (float*)M->U[r].p;
@@ -899,13 +1073,19 @@ int32_t sph_mlp_forward(void* model, void* stream, const sph_mlp_graph* gr, floa
         a.Cb = (__bf16*)M->HA[r + 1].p;
         a.ldcb = 2 * kH;
       }
-      SPH_TRY(gemm(st, false, false, a));
+      T.begin(1);
+      SPH_TRY(gemm(st, false, true, a));
+      T.end();
     }
   }
+  T.begin(3);
   hipLaunchKernelGGL(decode_kernel, dim3(nblk(n)), dim3(256), 0, st, (int)n,
                      (const float*)M->H[kR].p, M->w + L.w_out, M->w + L.b_out,
                      (const float*)M->N4.p, out_dev);
-  return launch_status("sph_mlp_forward");
+  T.end();
+  SPH_TRY(launch_status("sph_mlp_forward"));
+  T.collect();
+  return SPH_OK;
 }
 
 int32_t sph_mlp_loss(void* model, void* stream, const sph_mlp_graph* gr, const float* out_dev,
@@ -1055,7 +1235,8 @@ int32_t sph_mlp_adam(void* model, void* stream, float lr, float b1, float b2, fl
   const float c1 = 1.f - powf(b1, (float)t), c2 = 1.f - powf(b2, (float)t);
   hipLaunchKernelGGL(adam_kernel, dim3(nblk(M->L.total)), dim3(256), 0, as_stream(stream),
                      M->L.total, M->w, M->wb, M->g, M->m, M->v, lr, b1, b2, eps, c1, c2);
-  return launch_status("sph_mlp_adam");
+  SPH_TRY(launch_status("sph_mlp_adam"));
+  return refresh_transposed(M, as_stream(stream));
 }
 
 int32_t sph_mlp_integrate(void* stream, int64_t n, const int32_t* btype, const float* out_dev,

@@ -1,6 +1,7 @@
 """Public names of the solver layer (reference: solver/api.py)."""
 from . import generators  # noqa: F401
-from .integrator import EulerIntegrator, Integrator, RK2Integrator  # noqa: F401
+from .integrator import (EulerIntegrator, Integrator, LeapFrogIntegrator,  # noqa: F401
+                         PredictorCorrectorIntegrator, RK2Integrator, RK4Integrator)
 from .solver import Solver  # noqa: F401
 from .sph_equation import SPHIntegration, SPHOperation  # noqa: F401
 from .wcsph import wcsph_solver  # noqa: F401

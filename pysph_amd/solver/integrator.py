@@ -17,6 +17,8 @@ generic path.
 """
 import logging
 
+import torch
+
 from ..sph import _ops
 
 logger = logging.getLogger(__name__)
@@ -215,3 +217,142 @@ class RK2Integrator(Integrator):
         self.cstep = 1
         self._moved()
         self.particles.update()
+
+
+class RK4Integrator(Integrator):
+    """Classical RK4: K1 = F(X), K2 = F(X + h/2 K1), K3 = F(X + h/2 K2),
+    K4 = F(X + h K3), X += h/6 (K1 + 2 K2 + 2 K3 + K4)
+    (integrator.py:650-724)."""
+
+    def __init__(self, particles=None, calcs=()):
+        Integrator.__init__(self, particles, calcs)
+        self.nsteps = 4
+        self.fused = False
+
+    def final_step(self, calc, dt):
+        """`p = p_0 + dt/6*(k1 + 2*k2 + 2*k3 + k4); p_0 = p`
+        (integrator.py:669-689), summed in the reference's order."""
+        pa = self.arrays[calc.dnum]
+        kp = self.k_props[calc.id]
+        tmp = []
+        partial = []
+        for j, prop in enumerate(calc.updates):
+            t = torch.empty_like(pa.get_carray(prop))
+            tmp.append(t)
+            partial.append((t, pa.get_carray(kp["k1"][j]), pa.get_carray(kp["k2"][j]),
+                            pa.get_carray(kp["k3"][j])))
+        self._apply(pa, 5, 0.0, partial)
+        ops = []
+        for j, prop in enumerate(calc.updates):
+            ops.append((pa.get_carray(prop), pa.get_carray(self.initial_props[calc.id][j]),
+                        tmp[j], pa.get_carray(kp["k4"][j])))
+        self._apply(pa, 1, dt / 6.0, ops)
+
+    def integrate(self, dt):
+        self.set_initial_arrays()
+        self.do_step(self.calcs, 0.5 * dt)
+        self._moved()
+        self.particles.update()
+        self.do_step(self.calcs, 0.5 * dt)
+        self._moved()
+        self.particles.update()
+        self.do_step(self.calcs, dt)
+        self._moved()
+        self.particles.update()
+        self.eval(self.calcs)
+        for calc in self.calcs:
+            if calc.integrates:
+                self.final_step(calc, dt)
+        self.cstep = 1
+        self._moved()
+        self.particles.update()
+
+
+class PredictorCorrectorIntegrator(Integrator):
+    """Predict X(t + h/2) = X + h/2 F(X), correct X(t + h/2) = X + h/2
+    F(X(t + h/2)), then X(t + h) = 2 X(t + h/2) - X (integrator.py:726-798)."""
+
+    def __init__(self, particles=None, calcs=()):
+        Integrator.__init__(self, particles, calcs)
+        self.nsteps = 2
+        self.fused = False
+
+    def final_step(self, calc, dt):
+        """`p = 2*p - p_0; p_0 = p` (integrator.py:756-772)."""
+        pa = self.arrays[calc.dnum]
+        ops, copies = [], []
+        for j, prop in enumerate(calc.updates):
+            cur = pa.get_carray(prop)
+            init = pa.get_carray(self.initial_props[calc.id][j])
+            ops.append((cur, cur, init, None))
+            copies.append((init, cur, None, None))
+        self._apply(pa, 7, 0.0, ops)
+        self._apply(pa, 3, 0.0, copies)
+
+    def integrate(self, dt):
+        self.set_initial_arrays()
+        self.do_step(self.calcs, 0.5 * dt)
+        self._moved()
+        self.particles.update()
+        self.do_step(self.calcs, 0.5 * dt)
+        self._moved()
+        self.particles.update()
+        for calc in self.calcs:
+            if calc.integrates:
+                self.final_step(calc, dt)
+        self._moved()
+        self.particles.update()
+        self.cstep = 1
+
+
+class LeapFrogIntegrator(Integrator):
+    """Leap frog (integrator.py:800-912): step the non-position calcs, then
+    the position calcs with the new velocities, add 0.5 h^2 F_0 to the
+    positions, re-evaluate and correct v += 0.5 h (F - F_0)."""
+
+    def __init__(self, particles=None, calcs=()):
+        Integrator.__init__(self, particles, calcs)
+        self.nsteps = 2
+        self.fused = False
+
+    def add_correction_for_position(self, dt):
+        """positions += 0.5*dt*dt * k1 of each velocity calc
+        (integrator.py:829-858)."""
+        if not self.pcalcs:
+            return
+        pos_calc = self.pcalcs[0]
+        ppa = self.arrays[pos_calc.dnum]
+        for calc in self.icalcs:
+            if calc.tag != "velocity":
+                continue
+            pa = self.particles.arrays[calc.dnum]
+            ops = []
+            for j in range(len(calc.updates)):
+                cur = ppa.get_carray(pos_calc.updates[j])
+                ops.append((cur, cur, pa.get_carray(self.k_props[calc.id]["k1"][j]), None))
+            self._apply(ppa, 0, 0.5 * dt * dt, ops)
+        ppa.set_dirty(True)
+
+    def final_step(self, calc, dt):
+        """`p = p + 0.5*dt*(k2 - k1)` (integrator.py:860-883)."""
+        pa = self.arrays[calc.dnum]
+        ops = []
+        for j, prop in enumerate(calc.updates):
+            cur = pa.get_carray(prop)
+            ops.append((cur, cur, pa.get_carray(self.k_props[calc.id]["k2"][j]),
+                        pa.get_carray(self.k_props[calc.id]["k1"][j])))
+        self._apply(pa, 8, 0.5 * dt, ops)
+
+    def integrate(self, dt):
+        self.set_initial_arrays()
+        self.do_step(self.ncalcs, dt)
+        self.cstep = 1
+        self.do_step(self.pcalcs, dt)
+        self.add_correction_for_position(dt)
+        self._moved()
+        self.particles.barrier()
+        self.particles.update()
+        self.eval(self.ncalcs)
+        for calc in self.icalcs:
+            self.final_step(calc, dt)
+        self.cstep = 1

@@ -6,6 +6,7 @@
   XSPHCorrection    sph/funcs/xsph_funcs.pyx:8-88
   PositionStepping  sph/funcs/position_funcs.pyx:10-60
   GravityForce      sph/funcs/external_force.pyx:14-63
+  MoveCircleX/Y     sph/funcs/external_force.pyx:99-162
 
 Constructor signatures, ids, tags and num_outputs follow the reference so
 that operation set-up code written for it works unchanged.
@@ -157,3 +158,42 @@ class GravityForce(SPHFunction):
         for o, gc in zip(outputs, g):
             _ops.update(pa.dtype, pa.get_number_of_particles(), 4, gc, [(o, None, None, None)],
                         tag=pa.get_carray("tag"))
+
+
+class _MoveCircle(SPHFunction):
+    axis = 0
+
+    def __init__(self, source, dest=None, setup_arrays=True, *args, **kwargs):
+        SPHFunction.__init__(self, source, dest, setup_arrays)
+        self.tag = "position"
+        self.num_outputs = 2
+
+    def set_src_dst_reads(self):
+        self.src_reads = []
+        self.dst_reads = ["x", "y", "z", "tag"]
+
+    @classmethod
+    def calc_eval(cls, calc, outputs):
+        pa = calc.dest
+        o = list(outputs) + [None] * (2 - len(outputs))
+        stream = _ops.require_gpu(pa.get_carray("x"))
+        _abi.call("sph_move_circle", stream, pa.view(), cls.axis, _ops.ptr(o[0]), _ops.ptr(o[1]))
+
+
+class MoveCircleX(_MoveCircle):
+    """(-sin(angle), 0) with angle = arccos(x/|p|), sign flipped for y < 0:
+    the x velocity of unit-speed motion on a circle."""
+    axis = 0
+
+    def __init__(self, source, dest=None, setup_arrays=True, *args, **kwargs):
+        _MoveCircle.__init__(self, source, dest, setup_arrays)
+        self.id = "circlex"
+
+
+class MoveCircleY(_MoveCircle):
+    """(0, cos(angle)): the y velocity of unit-speed motion on a circle."""
+    axis = 1
+
+    def __init__(self, source, dest=None, setup_arrays=True, *args, **kwargs):
+        _MoveCircle.__init__(self, source, dest, setup_arrays)
+        self.id = "circley"

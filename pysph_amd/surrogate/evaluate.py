@@ -20,7 +20,7 @@ def _density(fluid, boundary, info):
 
 
 def compare(model, seed, nsteps=100, n=30000, dt_factor=4, check_every=10, threshold=0.01,
-            warm=0, log=None):
+            warm=0, log=None, baseline=None):
     """Run the three methods from the same initial state for
     nsteps * dt_factor plain steps (after `warm` plain steps of set-up) and
     return errors and timings."""
@@ -69,6 +69,21 @@ def compare(model, seed, nsteps=100, n=30000, dt_factor=4, check_every=10, thres
                                                info["dx"]).item()),
                checks=sur.stats["checks"], fallbacks=sur.stats["fallbacks"],
                check_errors=[round(e, 6) for e in sur.stats["errors"]])
+    # the same surrogate stepping without the learned correction
+    if baseline is not None:
+        f4, b4, info4 = scene()
+        base = SurrogateWCSPH(f4, b4, info4, baseline, dt_factor=dt_factor,
+                              check_every=check_every, threshold=threshold)
+        base.run(nsteps)
+        stb = base.state()
+        rho_b = base.density()[:nf]
+        x_b = torch.stack([stb["x"][:nf], stb["y"][:nf]])
+        out.update(untrained_density_error=float(((rho_b - rho_ref).abs().mean() /
+                                                  info["rho0"]).item()),
+                   untrained_position_error=float(((x_b - x_ref).norm(dim=0).mean() /
+                                                   info["dx"]).item()),
+                   untrained_fallbacks=base.stats["fallbacks"],
+                   untrained_check_errors=[round(e, 6) for e in base.stats["errors"]])
     # plain at dt_factor x dt
     f3, b3, info3 = scene()
     s3 = plain_solver(f3, b3, info3)

@@ -33,7 +33,8 @@ def init_params(seed=0):
         vec(H)
         mat(2 * H, H)
         vec(H)
-    mat(H, 2, 0.1)
+    # zero decoder: the untrained network is the plain SPH acceleration
+    mat(H, 2, 0.0)
     # output bias: gravity (0, -1) in units of g, so that the untrained
     # network is the plain SPH acceleration
     parts.append(np.array([0.0, -1.0, 0.0, 0.0]))
@@ -90,6 +91,14 @@ class EdgeMLP:
             m = cls(params=f["params"])
             m.meta = {k: f[k] for k in f.files if k != "params"}
         return m
+
+    def timing(self, enable=-1):
+        """Switch forward-pass launch timing on (1) / off (0), or only read
+        (-1).  Returns ({features, gemm, edges, decode}: ms total, calls)."""
+        ms = (ctypes.c_double * 4)()
+        calls = ctypes.c_int64(0)
+        _abi.call("sph_mlp_timing", self.handle, int(enable), ms, ctypes.byref(calls))
+        return dict(zip(("features", "gemm", "edges", "decode"), list(ms))), calls.value
 
     # ---- compute
     def forward(self, graph, out=None):

@@ -135,7 +135,7 @@ def main(argv=None):
     ap.add_argument("--bases", type=int, default=20)
     ap.add_argument("--spacing", type=int, default=100)
     ap.add_argument("--epochs", type=int, default=6)
-    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr", type=float, default=3e-4)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--val-bases", type=int, default=2)
     ap.add_argument("--n-min", type=int, default=20000)
@@ -157,8 +157,9 @@ def main(argv=None):
                seed=a.seed, dt_factor=4)
     log("saved %s (%.1f s total)" % (a.out, time.time() - t0))
     from .evaluate import compare
+    baseline = EdgeMLP(seed=a.seed)
     for es in (a.seed + 5000, a.seed + 5001):
-        r = compare(model, es, nsteps=a.eval_steps, n=a.eval_n, warm=2000)
+        r = compare(model, es, nsteps=a.eval_steps, n=a.eval_n, warm=2000, baseline=baseline)
         log(json.dumps(r))
 
 

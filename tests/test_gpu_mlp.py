@@ -56,6 +56,9 @@ def _params(seed):
     for k in p:
         if p[k].ndim == 1:
             p[k] = rng.normal(0, 0.1, p[k].shape)
+    # the trained decoder starts at zero; give it weights so that every
+    # layer receives gradient
+    p["w_out"] = rng.normal(0, 0.1, p["w_out"].shape)
     return M.flatten(p).astype(np.float32)
 
 

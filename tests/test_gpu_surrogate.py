@@ -1,0 +1,69 @@
+"""The trained edge-MLP surrogate on held-out 2D dam breaks (seeds disjoint
+from the training and validation seeds 0 and 1000).
+
+- A rollout of 4x plain steps, checked against the plain step every 10
+  steps, stays within 1% mean density error of the plain solver at dt, and
+  is far more accurate than the plain solver run at 4x dt (the method
+  without the network).
+- The trained network predicts held-out 4-dt accelerations better than the
+  untrained one (the SPH acceleration alone).
+- A surrogate whose check fails falls back to the plain step.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from pysph_amd.surrogate import evaluate as E
+from pysph_amd.surrogate import train as T
+from pysph_amd.surrogate.model import EdgeMLP, init_params
+from pysph_amd.surrogate.solver import SurrogateWCSPH
+
+pytestmark = pytest.mark.gpu
+
+WEIGHTS = os.path.join(os.path.dirname(T.__file__), "weights", "dam2d_edge_mlp.npz")
+
+
+def test_weights_are_committed():
+    assert os.path.exists(WEIGHTS)
+    m = EdgeMLP.load(WEIGHTS)
+    assert int(m.meta["rollouts"]) == 2000
+
+
+def test_surrogate_rollout_within_one_percent_and_beats_plain_at_4dt():
+    model = EdgeMLP.load(WEIGHTS)
+    # 100 steps of 4 dt: long enough for the plain step at 4 dt to drift
+    r = E.compare(model, 7001, nsteps=100, n=20000, warm=1000, baseline=EdgeMLP())
+    print(r)
+    assert r["surrogate_density_error"] <= 0.01
+    assert r["fallbacks"] == 0
+    assert np.isfinite(r["surrogate_position_error"])
+    # the plain step at 4 dt: diverges, or is at least 10x less accurate
+    assert (not r["big_dt_finite"]) or \
+        r["big_dt_density_error"] > 10 * r["surrogate_density_error"]
+
+
+def test_trained_network_improves_held_out_predictions():
+    data = T.generate(12, 1, 9001, spacing=100, n_range=(20000, 20000), warmup=1000,
+                      log=lambda s: None)
+    e_tr = T.evaluate(EdgeMLP.load(WEIGHTS), data)
+    e_un = T.evaluate(EdgeMLP(), data)
+    print("held-out relative acceleration error: trained %.4f, untrained %.4f"
+          % (e_tr["rel_err"], e_un["rel_err"]))
+    assert e_tr["rel_err"] < e_un["rel_err"]
+
+
+def test_failed_check_falls_back_to_the_plain_step():
+    # a random decoder: per-particle garbage accelerations, so the check
+    # after 10 steps sees the density field torn apart
+    theta = init_params(0)
+    theta[-260:-4] = np.random.Generator(np.random.PCG64(3)).normal(0, 5.0, 256)
+    rng = np.random.Generator(np.random.PCG64(7002))
+    f, b, info = T.dam_scene(rng, (20000, 20000))
+    sur = SurrogateWCSPH(f, b, info, EdgeMLP(params=theta), check_every=10, threshold=0.01)
+    sur.run(10)
+    assert sur.stats["checks"] == 1 and sur.stats["fallbacks"] == 1
+    assert sur.stats["errors"][0] > 0.01
+    assert sur.plain_left == 9
+    sur.run(9)
+    assert sur.stats["plain"] == 10

@@ -80,8 +80,8 @@ def test_physics_edge_features_sum_to_sph_accelerations():
     n = len(rho)
     btype[:] = 0
     X, E, dst, src = M.features(pos4, vel4, rho, btype, off, nbr, CONST)
-    # node features 6..9 are the sums; edge features 8..11 the scaled terms
-    fx = X[:, 6:10] * CONST["g"]
+    # node features 6..9 are the sums (x 0.01); edge features 8..11 the scaled terms
+    fx = X[:, 6:10] * 100.0 * CONST["g"]
     h = pos4[:, 3][dst]
     ps = 2.0 * CONST["g"] * h / CONST["u_scale"] ** 2
     fe = np.zeros((n, 4))
