@@ -97,6 +97,15 @@ typedef struct sph_params {
     double g[3];
     int32_t has_gravity;
     int32_t has_xsph;
+    /* 1: variable-h symmetric pairs (VarHNbrParticleLocator,
+     * nnps.pyx:654-749): a source interacts with a destination when
+     * d2 < (h_dst*radius_scale)^2 OR d2 < (h_src*radius_scale)^2, same
+     * distance arithmetic as the FixedDest test.  rmax must then be
+     * radius_scale * the largest h of the destination and the sources.
+     * The neighbour passes run their SoA kernels in this mode (packed
+     * inputs and neighbour lists are ignored); the batched engine rejects it. */
+    int32_t symmetric;
+    int32_t reserved;
 } sph_params;
 
 int32_t sph_abi_version(void);
@@ -160,6 +169,19 @@ int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
                           int64_t *nbrs_dev, int64_t cap, int64_t total,
                           int64_t *needed_host);
 
+/* ---- VarHNbrParticleLocator (nnps.pyx:654-749) as a CSR export: the
+ *      symmetric set d2 < (h_d*rs)^2 OR d2 < (h_s*rs)^2, rows sorted
+ *      ascending like the FixedDest export.  rmax = radius_scale * the
+ *      largest h of dst and src (it sizes the stencil). ---------------- */
+int32_t sph_varh_neighbor_counts(void *stream, const sph_grid *g, const sph_view *dst,
+                                 const sph_view *src, double radius_scale, double rmax,
+                                 int64_t *counts_dev);
+int32_t sph_varh_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
+                               const sph_view *src, double radius_scale, double rmax,
+                               const int64_t *src_ids_dev, const int64_t *offsets_dev,
+                               int64_t *nbrs_dev, int64_t cap, int64_t total,
+                               int64_t *needed_host);
+
 /* ---- neighbour passes -------------------------------------------------- */
 
 /* Optional neighbour list of one destination array, built by a density
@@ -199,6 +221,14 @@ int32_t sph_density(void *stream, const sph_grid *g, const sph_params *prm,
                     const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                     double rmax, const float *disp_dev, void *rho_dev,
                     void *p_dev, void *cs_dev, const sph_nlist *nlist_out);
+
+/* SPHDensityRate over all sources (density_funcs.pyx:89-158, hks off):
+ * rate_dev[a] = sum_b m_b (v_a - v_b) . grad_a W(r_a - r_b, (h_a+h_b)/2),
+ * the continuity-equation density derivative.  Reads x,y,z,h,u,v,w of the
+ * destination and x,y,z,h,u,v,w,m of the sources. */
+int32_t sph_density_rate(void *stream, const sph_grid *g, const sph_params *prm,
+                         const sph_view *dst, int32_t nsrc, const sph_view *srcs,
+                         double rmax, const float *disp_dev, void *rate_dev);
 
 /* TaitEquation alone: p, cs from rho (eos_funcs.pyx:105-116). */
 int32_t sph_tait(void *stream, const sph_params *prm, const sph_view *a,

@@ -66,7 +66,7 @@ def lib():
         L.orc_nnps_csr.restype = ctypes.c_int64
         L.orc_nnps_csr.argtypes = [ctypes.POINTER(OrcArr), ctypes.POINTER(OrcArr),
                                    ctypes.c_int, ctypes.c_double, ctypes.c_double,
-                                   _lp, _lp, ctypes.c_int64]
+                                   _lp, _lp, ctypes.c_int64, ctypes.c_int]
         L.orc_point_query.restype = ctypes.c_int64
         L.orc_point_query.argtypes = [ctypes.POINTER(OrcArr), ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double,
@@ -76,6 +76,8 @@ def lib():
         L.orc_nbrs_build.argtypes = [ctypes.POINTER(OrcNbrs), ctypes.POINTER(OrcArr),
                                      ctypes.POINTER(OrcArr), ctypes.c_double,
                                      ctypes.c_double]
+        L.orc_nbrs_build_varh.restype = ctypes.c_int
+        L.orc_nbrs_build_varh.argtypes = L.orc_nbrs_build.argtypes
         L.orc_nbrs_free.argtypes = [ctypes.POINTER(OrcNbrs)]
         pa = ctypes.POINTER(ctypes.POINTER(OrcArr))
         pn = ctypes.POINTER(ctypes.POINTER(OrcNbrs))
@@ -88,6 +90,8 @@ def lib():
                                    ctypes.c_double, _dp, _dp, _dp]
         L.orc_xsph.argtypes = [ctypes.c_int, ctypes.POINTER(OrcArr), ctypes.c_int,
                                pa, pn, ctypes.c_double, _dp, _dp, _dp]
+        L.orc_density_rate.argtypes = [ctypes.c_int, ctypes.POINTER(OrcArr), ctypes.c_int,
+                                       pa, pn, _dp]
         L.orc_scheme_step.restype = ctypes.c_int
         L.orc_scheme_step.argtypes = [ctypes.POINTER(OrcScheme), ctypes.c_int, pa]
         L.orc_scheme_rhs.restype = ctypes.c_int
@@ -160,8 +164,9 @@ def real_to_int(val, step):
     return lib().orc_real_to_int(val, step)
 
 
-def nnps_csr(dst, src=None, cell_size=None, radius_scale=2.0):
-    """FixedDest neighbour lists of dst from src (self last when src is dst).
+def nnps_csr(dst, src=None, cell_size=None, radius_scale=2.0, varh=False):
+    """FixedDest neighbour lists of dst from src (self last when src is dst),
+    or with varh the symmetric variable-h lists of VarHNbrParticleLocator.
     Returns (offsets int64[n+1], nbrs int64[total])."""
     same = src is None or src is dst
     if cell_size is None:
@@ -175,13 +180,14 @@ def nnps_csr(dst, src=None, cell_size=None, radius_scale=2.0):
     cap = 0
     nb = np.zeros(1, dtype=np.int64)
     total = lib().orc_nnps_csr(ctypes.byref(bd.s), ctypes.byref(bs.s), int(same),
-                               cell_size, radius_scale, _ptr(off, _lp), _ptr(nb, _lp), cap)
+                               cell_size, radius_scale, _ptr(off, _lp), _ptr(nb, _lp), cap,
+                               int(varh))
     if total < 0:
         raise MemoryError("oracle neighbour build failed")
     nb = np.zeros(max(total, 1), dtype=np.int64)
     total = lib().orc_nnps_csr(ctypes.byref(bd.s), ctypes.byref(bs.s), int(same),
                                cell_size, radius_scale, _ptr(off, _lp), _ptr(nb, _lp),
-                               total)
+                               total, int(varh))
     return off, nb[:total]
 
 
@@ -196,13 +202,14 @@ def point_query(src, cell_size, pnt, radius, exclude_index=-1):
 class _Calc:
     """Binds dst and srcs and builds the per-(src,dst) neighbour caches."""
 
-    def __init__(self, dst, srcs, cell_size, radius_scale):
+    def __init__(self, dst, srcs, cell_size, radius_scale, varh=False):
         self.bd = _Bound(dst)
         self.bs = [self.bd if s is dst else _Bound(s) for s in srcs]
         self.nb = [OrcNbrs() for _ in srcs]
+        build = lib().orc_nbrs_build_varh if varh else lib().orc_nbrs_build
         for nb, b in zip(self.nb, self.bs):
-            if lib().orc_nbrs_build(ctypes.byref(nb), ctypes.byref(self.bd.s),
-                                    ctypes.byref(b.s), cell_size, radius_scale):
+            if build(ctypes.byref(nb), ctypes.byref(self.bd.s), ctypes.byref(b.s), cell_size,
+                     radius_scale):
                 raise MemoryError("oracle neighbour build failed")
         k = len(srcs)
         self.pa = (ctypes.POINTER(OrcArr) * k)(*[ctypes.pointer(b.s) for b in self.bs])
@@ -221,9 +228,9 @@ def _cell_size(arrs, radius_scale, cell_size):
     return radius_scale * mh or 1.0
 
 
-def sphrho(dim, dst, srcs, cell_size=None, radius_scale=2.0):
+def sphrho(dim, dst, srcs, cell_size=None, radius_scale=2.0, varh=False):
     cs = _cell_size([dst] + list(srcs), radius_scale, cell_size)
-    c = _Calc(dst, srcs, cs, radius_scale)
+    c = _Calc(dst, srcs, cs, radius_scale, varh)
     out = np.zeros(len(dst["x"]))
     lib().orc_sphrho(dim, ctypes.byref(c.bd.s), c.k, c.pa, c.pn, _ptr(out))
     c.close()
@@ -239,9 +246,9 @@ def tait(arr, co, ro, gamma):
 
 
 def momentum(dim, dst, srcs, alpha=1.0, beta=1.0, eta=0.1, cell_size=None,
-             radius_scale=2.0):
+             radius_scale=2.0, varh=False):
     cs = _cell_size([dst] + list(srcs), radius_scale, cell_size)
-    c = _Calc(dst, srcs, cs, radius_scale)
+    c = _Calc(dst, srcs, cs, radius_scale, varh)
     n = len(dst["x"])
     o = [np.zeros(n) for _ in range(3)]
     lib().orc_momentum(dim, ctypes.byref(c.bd.s), c.k, c.pa, c.pn, alpha, beta, eta,
@@ -250,15 +257,25 @@ def momentum(dim, dst, srcs, alpha=1.0, beta=1.0, eta=0.1, cell_size=None,
     return np.stack(o)
 
 
-def xsph(dim, dst, srcs, eps=0.5, cell_size=None, radius_scale=2.0):
+def xsph(dim, dst, srcs, eps=0.5, cell_size=None, radius_scale=2.0, varh=False):
     cs = _cell_size([dst] + list(srcs), radius_scale, cell_size)
-    c = _Calc(dst, srcs, cs, radius_scale)
+    c = _Calc(dst, srcs, cs, radius_scale, varh)
     n = len(dst["x"])
     o = [np.zeros(n) for _ in range(3)]
     lib().orc_xsph(dim, ctypes.byref(c.bd.s), c.k, c.pa, c.pn, eps,
                    _ptr(o[0]), _ptr(o[1]), _ptr(o[2]))
     c.close()
     return np.stack(o)
+
+
+def density_rate(dim, dst, srcs, cell_size=None, radius_scale=2.0, varh=False):
+    """SPHDensityRate over srcs (density_funcs.pyx:89-158)."""
+    cs = _cell_size([dst] + list(srcs), radius_scale, cell_size)
+    c = _Calc(dst, srcs, cs, radius_scale, varh)
+    out = np.zeros(len(dst["x"]))
+    lib().orc_density_rate(dim, ctypes.byref(c.bd.s), c.k, c.pa, c.pn, _ptr(out))
+    c.close()
+    return out
 
 
 def scheme(dim, integrator="rk2", dt=1e-5, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0,

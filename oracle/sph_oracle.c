@@ -22,6 +22,9 @@
  *   TaitEquation .......... sph/funcs/eos_funcs.pyx:84-116
  *   MomentumEquation ...... sph/funcs/pressure_funcs.pyx:154-245
  *   XSPHCorrection ........ sph/funcs/xsph_funcs.pyx:31-88
+ *   SPHDensityRate ........ sph/funcs/density_funcs.pyx:89-158
+ *   VarH locator .......... base/nnps.pyx:654-749 (forward query, then the
+ *                           reverse locator's hits appended if absent)
  *   PositionStepping ...... sph/funcs/position_funcs.pyx:32-60
  *   GravityForce .......... sph/funcs/external_force.pyx:39-45
  *   Euler / RK2 ........... solver/integrator.py:339-514 (set_initial_arrays,
@@ -309,6 +312,79 @@ int orc_nbrs_build(orc_nbrs *out, const orc_arr *dst, const orc_arr *src,
     return 0;
 }
 
+/* VarHNbrParticleLocator._update_cache (nnps.pyx:654-749): the row of
+ * destination k is its FixedDest query without the self swap, followed by
+ * every source j (ascending) whose own query over the destination array,
+ * with radius h_j * radius_scale, finds k and that the row does not hold
+ * yet.  The union is the symmetric set: d2 < (h_k rs)^2 or d2 < (h_j rs)^2.
+ * Returns 0 or -1 on allocation failure. */
+int orc_nbrs_build_varh(orc_nbrs *out, const orc_arr *dst, const orc_arr *src,
+                        double cell_size, double radius_scale)
+{
+    orc_cells cs_src, cd;
+    int64_t i, j, total = 0, rtotal = 0;
+    int64_t *foff, *fnbr, *roff, *rnbr, *eoff, *ecnt, *enbr;
+    if (cell_size <= 0) return -1;
+    if (cells_build(&cs_src, src, cell_size)) return -1;
+    if (cells_build(&cd, dst, cell_size)) return -1;
+    foff = (int64_t *)calloc((size_t)dst->n + 1, sizeof(int64_t));
+    roff = (int64_t *)calloc((size_t)src->n + 1, sizeof(int64_t));
+    if (!foff || !roff) return -1;
+    /* forward rows: FixedDest queries of dst from src */
+    for (i = 0; i < dst->n; ++i)
+        foff[i + 1] = foff[i] + query_one(&cs_src, src, cell_size, dst->x[i], dst->y[i],
+                                          dst->z[i], dst->h[i] * radius_scale, NULL, 0, 0);
+    fnbr = (int64_t *)malloc(sizeof(int64_t) * (foff[dst->n] + 1));
+    if (!fnbr) return -1;
+    for (i = 0; i < dst->n; ++i)
+        query_one(&cs_src, src, cell_size, dst->x[i], dst->y[i], dst->z[i],
+                  dst->h[i] * radius_scale, fnbr + foff[i], 0, foff[i + 1] - foff[i]);
+    /* reverse rows: queries of src from dst with the source's radius */
+    for (j = 0; j < src->n; ++j)
+        roff[j + 1] = roff[j] + query_one(&cd, dst, cell_size, src->x[j], src->y[j],
+                                          src->z[j], src->h[j] * radius_scale, NULL, 0, 0);
+    rtotal = roff[src->n];
+    rnbr = (int64_t *)malloc(sizeof(int64_t) * (rtotal + 1));
+    if (!rnbr) return -1;
+    for (j = 0; j < src->n; ++j)
+        query_one(&cd, dst, cell_size, src->x[j], src->y[j], src->z[j],
+                  src->h[j] * radius_scale, rnbr + roff[j], 0, roff[j + 1] - roff[j]);
+    /* candidates appended to row k: the j (ascending) whose reverse row has k */
+    eoff = (int64_t *)calloc((size_t)dst->n + 1, sizeof(int64_t));
+    ecnt = (int64_t *)calloc((size_t)dst->n + 1, sizeof(int64_t));
+    enbr = (int64_t *)malloc(sizeof(int64_t) * (rtotal + 1));
+    if (!eoff || !ecnt || !enbr) return -1;
+    for (i = 0; i < rtotal; ++i) eoff[rnbr[i] + 1]++;
+    for (i = 0; i < dst->n; ++i) eoff[i + 1] += eoff[i];
+    for (j = 0; j < src->n; ++j)
+        for (i = roff[j]; i < roff[j + 1]; ++i) {
+            int64_t k = rnbr[i];
+            enbr[eoff[k] + ecnt[k]++] = j;
+        }
+    /* merge: forward row, then the candidates it does not hold yet */
+    out->n_dst = dst->n;
+    out->off = (int64_t *)malloc(sizeof(int64_t) * (dst->n + 1));
+    out->nbr = (int64_t *)malloc(sizeof(int64_t) * (foff[dst->n] + rtotal + 1));
+    if (!out->off || !out->nbr) return -1;
+    out->off[0] = 0;
+    for (i = 0; i < dst->n; ++i) {
+        int64_t b = foff[i], e = foff[i + 1], t, u;
+        for (t = b; t < e; ++t) out->nbr[total++] = fnbr[t];
+        for (t = eoff[i]; t < eoff[i + 1]; ++t) {
+            int present = 0;
+            for (u = out->off[i]; u < total; ++u)
+                if (out->nbr[u] == enbr[t]) { present = 1; break; }
+            if (!present) out->nbr[total++] = enbr[t];
+        }
+        out->off[i + 1] = total;
+    }
+    free(foff); free(fnbr); free(roff); free(rnbr);
+    free(eoff); free(ecnt); free(enbr);
+    cells_free(&cs_src);
+    cells_free(&cd);
+    return 0;
+}
+
 void orc_nbrs_free(orc_nbrs *nb)
 {
     free(nb->off);
@@ -320,12 +396,13 @@ void orc_nbrs_free(orc_nbrs *nb)
  * cap suffices.  offsets must hold n_dst + 1 entries. */
 int64_t orc_nnps_csr(const orc_arr *dst, const orc_arr *src, int same,
                      double cell_size, double radius_scale, int64_t *offsets,
-                     int64_t *nbrs, int64_t cap)
+                     int64_t *nbrs, int64_t cap, int varh)
 {
     orc_nbrs nb;
     int64_t total, i;
     const orc_arr *s = same ? dst : src;
-    if (orc_nbrs_build(&nb, dst, s, cell_size, radius_scale)) return -1;
+    if (varh ? orc_nbrs_build_varh(&nb, dst, s, cell_size, radius_scale)
+             : orc_nbrs_build(&nb, dst, s, cell_size, radius_scale)) return -1;
     total = nb.off[dst->n];
     for (i = 0; i <= dst->n; ++i) offsets[i] = nb.off[i];
     if (total <= cap) memcpy(nbrs, nb.nbr, sizeof(int64_t) * total);
@@ -508,6 +585,37 @@ void orc_xsph(int dim, const orc_arr *dst, int nsrc, const orc_arr *const *srcs,
             o1[i] += nr[0];
             if (dim > 1) o2[i] += nr[1];
             if (dim > 2) o3[i] += nr[2];
+        }
+    }
+}
+
+/* SPHDensityRate: density_funcs.pyx:89-158 (hks off, no corrections):
+ * drho_a/dt = sum_b m_b (v_a - v_b) . grad_a W(r_a - r_b, h_ab). */
+void orc_density_rate(int dim, const orc_arr *dst, int nsrc, const orc_arr *const *srcs,
+                      const orc_nbrs *const *nb, double *out)
+{
+    int64_t i;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(g_threads)
+    for (i = 0; i < dst->n; ++i) {
+        int s;
+        out[i] = 0.0;
+        if (!is_local(dst, i)) continue;
+        for (s = 0; s < nsrc; ++s) {
+            const orc_arr *src = srcs[s];
+            double nr = 0.0;
+            int64_t t;
+            for (t = nb[s]->off[i]; t < nb[s]->off[i + 1]; ++t) {
+                int64_t j = nb[s]->nbr[t];
+                double hab = 0.5 * (dst->h[i] + src->h[j]);
+                double vx = dst->u[i] - src->u[j], vy = dst->v[i] - src->v[j],
+                       vz = dst->w[i] - src->w[j];
+                double g[3];
+                orc_cubic_grad(dst->x[i], dst->y[i], dst->z[i], src->x[j], src->y[j],
+                               src->z[j], hab, dim, g);
+                /* cPoint_dot(vel, grad) * m_b */
+                nr += (vx * g[0] + vy * g[1] + vz * g[2]) * src->m[j];
+            }
+            out[i] += nr;
         }
     }
 }

@@ -48,7 +48,8 @@ class SphParams(ctypes.Structure):
                 ("alpha", ctypes.c_double), ("beta", ctypes.c_double),
                 ("eta", ctypes.c_double), ("eps", ctypes.c_double),
                 ("g", ctypes.c_double * 3), ("has_gravity", ctypes.c_int32),
-                ("has_xsph", ctypes.c_int32)]
+                ("has_xsph", ctypes.c_int32), ("symmetric", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 class SphNlist(ctypes.Structure):
@@ -137,6 +138,15 @@ _SIGS = {
     "sph_density": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                      ctypes.c_int32, _P(SphView), ctypes.c_double, _vp, _vp,
                                      _vp, _vp, _P(SphNlist)]),
+    "sph_varh_neighbor_counts": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
+                                                  ctypes.c_double, ctypes.c_double, _vp]),
+    "sph_varh_neighbor_fill": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
+                                                ctypes.c_double, ctypes.c_double, _vp, _vp,
+                                                _vp, ctypes.c_int64, ctypes.c_int64,
+                                                _P(ctypes.c_int64)]),
+    "sph_density_rate": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
+                                          ctypes.c_int32, _P(SphView), ctypes.c_double, _vp,
+                                          _vp]),
     "sph_tait": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp, _vp]),
     "sph_momentum_xsph": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                            ctypes.c_int32, _P(SphView), _P(ctypes.c_uint32),

@@ -4,7 +4,7 @@ from .cell import (CellManager, construct_immediate_neighbor_list, find_cell_id,
                    py_find_cell_id, py_real_to_int, real_to_int)
 from .kernels import CubicSplineKernel, DummyKernel, KernelBase  # noqa: F401
 from .nnps import (FixedDestNbrParticleLocator, NbrParticleLocatorBase,  # noqa: F401
-                   NeighborLocatorType, NNPSManager, neighbor_csr)
+                   NeighborLocatorType, NNPSManager, VarHNbrParticleLocator, neighbor_csr)
 from .particle_array import ParticleArray, get_particle_array  # noqa: F401
 from .particle_types import (GhostParticle, LocalDummy, LocalReal, ParticleType,  # noqa: F401
                              RemoteDummy, RemoteReal)

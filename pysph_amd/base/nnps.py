@@ -45,9 +45,11 @@ def _csr_user_order(pa_dst, off_slot, nbr, n):
     return uoff, out
 
 
-def neighbor_csr(cell_manager, src, dst, radius_scale):
+def neighbor_csr(cell_manager, src, dst, radius_scale, symmetric=False):
     """All neighbour lists of dst from src on the GPU: (offsets, nbrs) in user
-    index space, rows in user order, each row sorted ascending."""
+    index space, rows in user order, each row sorted ascending.  With
+    `symmetric` the rows are the variable-h sets of VarHNbrParticleLocator
+    (nnps.pyx:654-749): d2 < (h_dst rs)^2 or d2 < (h_src rs)^2."""
     cm = cell_manager
     cm.update()
     n = dst.get_number_of_particles()
@@ -58,8 +60,13 @@ def neighbor_csr(cell_manager, src, dst, radius_scale):
     counts = torch.empty(n, dtype=torch.int64, device=dev)
     vd = dst.view()
     vs = src.view()
-    _abi.call("sph_neighbor_counts", stream, cm.grid, vd, vs, float(radius_scale),
-              counts.data_ptr())
+    if symmetric:
+        rmax = float(radius_scale) * cm.max_h
+        _abi.call("sph_varh_neighbor_counts", stream, cm.grid, vd, vs, float(radius_scale),
+                  rmax, counts.data_ptr())
+    else:
+        _abi.call("sph_neighbor_counts", stream, cm.grid, vd, vs, float(radius_scale),
+                  counts.data_ptr())
     offsets = torch.empty(n + 1, dtype=torch.int64, device=dev)
     total = _abi.ctypes.c_int64(0)
     _abi.call("sph_scan_counts", stream, n, counts.data_ptr(), offsets.data_ptr(),
@@ -69,9 +76,14 @@ def neighbor_csr(cell_manager, src, dst, radius_scale):
     ids = src._order if src._order is not None else torch.arange(
         src.get_number_of_particles(), dtype=torch.int64, device=dev)
     needed = _abi.ctypes.c_int64(0)
-    _abi.call("sph_neighbor_fill", stream, cm.grid, vd, vs, float(radius_scale),
-              ids.data_ptr(), offsets.data_ptr(), nbrs.data_ptr(), total, total,
-              _abi.ctypes.byref(needed))
+    if symmetric:
+        _abi.call("sph_varh_neighbor_fill", stream, cm.grid, vd, vs, float(radius_scale),
+                  rmax, ids.data_ptr(), offsets.data_ptr(), nbrs.data_ptr(), total, total,
+                  _abi.ctypes.byref(needed))
+    else:
+        _abi.call("sph_neighbor_fill", stream, cm.grid, vd, vs, float(radius_scale),
+                  ids.data_ptr(), offsets.data_ptr(), nbrs.data_ptr(), total, total,
+                  _abi.ctypes.byref(needed))
     off = offsets.cpu().numpy()
     nb = nbrs[:total].cpu().numpy()
     return _csr_user_order(dst, off, nb, n)
@@ -137,7 +149,7 @@ class FixedDestNbrParticleLocator(NbrParticleLocatorBase):
                 self.is_dirty = True
         if self.is_dirty or self._off is None:
             off, nb = neighbor_csr(self.cell_manager, self.source, self.dest,
-                                   self.radius_scale)
+                                   self.radius_scale, symmetric=self.symmetric)
             if self.source is self.dest:
                 # keep the particle itself in the last slot (nnps.pyx:612-618)
                 for i in range(len(off) - 1):
@@ -173,6 +185,20 @@ class FixedDestNbrParticleLocator(NbrParticleLocatorBase):
 
     get_nearest_particles = py_get_nearest_particles
 
+    symmetric = False
+
+
+class VarHNbrParticleLocator(FixedDestNbrParticleLocator):
+    """Symmetric neighbour lists for variable smoothing lengths
+    (nnps.pyx:654-749): j is a neighbour of i when i's search finds j or j's
+    search (with h_j) finds i.  The reference builds the union by appending
+    the reverse locator's hits; here one GPU pass tests both radii per pair.
+    Rows hold the same sets, sorted, with the particle itself last when
+    source is destination (the reference leaves it where its forward search
+    put it)."""
+
+    symmetric = True
+
 
 class NNPSManager:
     """Hands out cached locators keyed by (src, dst, radius_scale)
@@ -180,27 +206,28 @@ class NNPSManager:
 
     def __init__(self, cell_manager=None, variable_h=False,
                  locator_type=NeighborLocatorType.SPHNeighborLocator):
-        if variable_h:
-            raise NotImplementedError("variable-h neighbour locators are not supported")
         self.cell_manager = cell_manager
         self.variable_h = variable_h
         self.locator_type = locator_type
         self.particle_locator_cache = {}
         self.polygon_cache = {}
 
-    def get_neighbor_particle_locator(self, src, dst=None, radius_scale=1.0):
-        if dst is None:
-            return NbrParticleLocatorBase(src, self.cell_manager, self.locator_type)
-        key = (id(src), id(dst), radius_scale)
+    def get_neighbor_particle_locator(self, source, dest=None, radius_scale=1.0):
+        """A base locator without dest, else the cached locator of
+        (source, dest, radius_scale) (nnps.pyx:789-807)."""
+        if dest is None:
+            return NbrParticleLocatorBase(source, self.cell_manager, self.locator_type)
+        key = (id(source), id(dest), radius_scale)
         loc = self.particle_locator_cache.get(key)
         if loc is None:
-            loc = FixedDestNbrParticleLocator(src, dst, radius_scale, self.cell_manager,
-                                              locator_type=self.locator_type)
+            cls = VarHNbrParticleLocator if self.variable_h else FixedDestNbrParticleLocator
+            loc = cls(source, dest, radius_scale, self.cell_manager,
+                      locator_type=self.locator_type)
             self.particle_locator_cache[key] = loc
         return loc
 
-    def add_interaction(self, src, dst, radius_scale):
-        self.get_neighbor_particle_locator(src, dst, radius_scale)
+    def add_interaction(self, source, dest, radius_scale):
+        self.get_neighbor_particle_locator(source, dest, radius_scale)
 
     def update(self):
         for loc in self.particle_locator_cache.values():

@@ -1,8 +1,10 @@
 """Particles: the arrays of a simulation with their cell and neighbour
 managers (reference: base/particles.py:23-280).
 
-MPI domain decomposition (in_parallel) and variable-h locators are not part
-of this single-GPU build; asking for them raises NotImplementedError.
+MPI domain decomposition (in_parallel) is not part of this single-GPU
+build; asking for it raises NotImplementedError.  variable_h=True gives the
+symmetric variable-h neighbour sets (VarHNbrParticleLocator) to the locators
+and to every SPH operation on these arrays.
 """
 from .cell import CellManager
 from .nnps import NeighborLocatorType, NNPSManager

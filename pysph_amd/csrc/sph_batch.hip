@@ -278,6 +278,10 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
       set_error("sph_batch_create: scene %d: rmax, radius_scale and dt must be positive", s);
       return SPH_EINVAL;
     }
+    if (S.prm.symmetric) {
+      set_error("sph_batch_create: scene %d: variable-h symmetric pairs are not supported", s);
+      return SPH_EUNSUPPORTED;
+    }
     f_end += S.nf;
     b_end += S.nb;
     ncells += scene_cells(S);

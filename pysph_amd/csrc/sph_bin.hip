@@ -307,6 +307,7 @@ struct NbrArgs {
   const int64_t* off;
   int64_t* out;
   int64_t* counts;
+  double rmax_sym;  // > 0: symmetric variable-h sets, stencil for this radius
 };
 
 template <typename T>
@@ -338,8 +339,9 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   const float r2f = (float)r2d;
   int cx, cy, cz;
   cell_xyz(a.dst.cell[i], g, cx, cy, cz);
-  // ring = ceil(r/cs) for this destination (cell.pyx:1127-1129)
-  const double rd = ceil(r / g.cs);
+  // ring = ceil(r/cs) for this destination (cell.pyx:1127-1129); the
+  // symmetric sets also reach sources whose own radius covers the pair
+  const double rd = ceil((a.rmax_sym > r ? a.rmax_sym : r) / g.cs);
   const int full = max(g.dims[0], max(g.dims[1], g.dims[2]));
   int ring = !(rd < (double)full) ? full : (int)rd;
   if (ring < 1) ring = 1;
@@ -347,7 +349,14 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   const View<T>& S = a.src;
   for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
     for (int j = j0; j < j1; ++j) {
-      if (pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d)) {
+      bool hit = pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d);
+      if (!hit && a.rmax_sym > 0) {
+        // the reverse locator's test: d2 < (h_s * rs)^2 (nnps.pyx:735-747)
+        const double rsj = (double)S.h[j] * a.rs;
+        const double r2s = rsj * rsj;
+        hit = pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, (float)r2s, r2s);
+      }
+      if (hit) {
         if (FILL) a.out[k] = a.ids[j];
         ++k;
       }
@@ -563,12 +572,13 @@ static int32_t nbr_common(const sph_grid* g, const sph_view* dst, const sph_view
   return SPH_OK;
 }
 
-int32_t sph_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst,
-                            const sph_view* src, double radius_scale, int64_t* counts_dev) {
-  SPH_TRY(nbr_common(g, dst, src, radius_scale, "sph_neighbor_counts"));
+static int32_t nbr_counts(void* stream, const sph_grid* g, const sph_view* dst,
+                          const sph_view* src, double radius_scale, double rmax_sym,
+                          int64_t* counts_dev, const char* who) {
+  SPH_TRY(nbr_common(g, dst, src, radius_scale, who));
   if (dst->n == 0) return SPH_OK;
   if (!counts_dev) {
-    set_error("sph_neighbor_counts: NULL output");
+    set_error("%s: NULL output", who);
     return SPH_EINVAL;
   }
   hipStream_t st = as_stream(stream);
@@ -576,14 +586,61 @@ int32_t sph_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst
   const int nb = nblocks_for(dst->n);
   if (dst->dtype == SPH_F32) {
     NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, nullptr,
-                     nullptr, nullptr, counts_dev};
+                     nullptr, nullptr, counts_dev, rmax_sym};
     hipLaunchKernelGGL((nbr_kernel<float, false>), dim3(nb), dim3(kBlock), 0, st, G, a);
   } else {
     NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, nullptr,
-                      nullptr, nullptr, counts_dev};
+                      nullptr, nullptr, counts_dev, rmax_sym};
     hipLaunchKernelGGL((nbr_kernel<double, false>), dim3(nb), dim3(kBlock), 0, st, G, a);
   }
-  return launch_status("sph_neighbor_counts");
+  return launch_status(who);
+}
+
+static int32_t nbr_fill(void* stream, const sph_grid* g, const sph_view* dst,
+                        const sph_view* src, double radius_scale, double rmax_sym,
+                        const int64_t* src_ids_dev, const int64_t* offsets_dev,
+                        int64_t* nbrs_dev, int64_t cap, int64_t total, int64_t* needed_host,
+                        const char* who) {
+  SPH_TRY(nbr_common(g, dst, src, radius_scale, who));
+  if (needed_host) *needed_host = total;
+  if (cap < total) {
+    set_error("%s: capacity %lld < %lld pairs", who, (long long)cap, (long long)total);
+    return SPH_ECAPACITY;
+  }
+  if (dst->n == 0 || total == 0) return SPH_OK;
+  if (!src_ids_dev || !offsets_dev || !nbrs_dev) {
+    set_error("%s: NULL argument", who);
+    return SPH_EINVAL;
+  }
+  hipStream_t st = as_stream(stream);
+  Grid G = make_grid(*g, 0.0);
+  const int nb = nblocks_for(dst->n);
+  if (dst->dtype == SPH_F32) {
+    NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, src_ids_dev,
+                     offsets_dev, nbrs_dev, nullptr, rmax_sym};
+    hipLaunchKernelGGL((nbr_kernel<float, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  } else {
+    NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, src_ids_dev,
+                      offsets_dev, nbrs_dev, nullptr, rmax_sym};
+    hipLaunchKernelGGL((nbr_kernel<double, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  }
+  return launch_status(who);
+}
+
+int32_t sph_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst,
+                            const sph_view* src, double radius_scale, int64_t* counts_dev) {
+  return nbr_counts(stream, g, dst, src, radius_scale, 0.0, counts_dev, "sph_neighbor_counts");
+}
+
+int32_t sph_varh_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst,
+                                 const sph_view* src, double radius_scale, double rmax,
+                                 int64_t* counts_dev) {
+  if (!(rmax > 0)) {
+    set_error("sph_varh_neighbor_counts: rmax must be positive");
+    return SPH_EINVAL;
+  }
+  return nbr_counts(stream, g, dst, src, radius_scale, rmax, counts_dev,
+                    "sph_varh_neighbor_counts");
 }
 
 int32_t sph_scan_counts(void* stream, int64_t n, const int64_t* counts_dev,
@@ -618,31 +675,21 @@ int32_t sph_neighbor_fill(void* stream, const sph_grid* g, const sph_view* dst,
                           const sph_view* src, double radius_scale, const int64_t* src_ids_dev,
                           const int64_t* offsets_dev, int64_t* nbrs_dev, int64_t cap,
                           int64_t total, int64_t* needed_host) {
-  SPH_TRY(nbr_common(g, dst, src, radius_scale, "sph_neighbor_fill"));
-  if (needed_host) *needed_host = total;
-  if (cap < total) {
-    set_error("sph_neighbor_fill: capacity %lld < %lld pairs", (long long)cap,
-              (long long)total);
-    return SPH_ECAPACITY;
-  }
-  if (dst->n == 0 || total == 0) return SPH_OK;
-  if (!src_ids_dev || !offsets_dev || !nbrs_dev) {
-    set_error("sph_neighbor_fill: NULL argument");
+  return nbr_fill(stream, g, dst, src, radius_scale, 0.0, src_ids_dev, offsets_dev, nbrs_dev,
+                  cap, total, needed_host, "sph_neighbor_fill");
+}
+
+int32_t sph_varh_neighbor_fill(void* stream, const sph_grid* g, const sph_view* dst,
+                               const sph_view* src, double radius_scale, double rmax,
+                               const int64_t* src_ids_dev, const int64_t* offsets_dev,
+                               int64_t* nbrs_dev, int64_t cap, int64_t total,
+                               int64_t* needed_host) {
+  if (!(rmax > 0)) {
+    set_error("sph_varh_neighbor_fill: rmax must be positive");
     return SPH_EINVAL;
   }
-  hipStream_t st = as_stream(stream);
-  Grid G = make_grid(*g, 0.0);
-  const int nb = nblocks_for(dst->n);
-  if (dst->dtype == SPH_F32) {
-    NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, src_ids_dev,
-                     offsets_dev, nbrs_dev, nullptr};
-    hipLaunchKernelGGL((nbr_kernel<float, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
-  } else {
-    NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, src_ids_dev,
-                      offsets_dev, nbrs_dev, nullptr};
-    hipLaunchKernelGGL((nbr_kernel<double, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
-  }
-  return launch_status("sph_neighbor_fill");
+  return nbr_fill(stream, g, dst, src, radius_scale, rmax, src_ids_dev, offsets_dev, nbrs_dev,
+                  cap, total, needed_host, "sph_varh_neighbor_fill");
 }
 
 }  // extern "C"

@@ -77,6 +77,8 @@ struct PassArgs {
   int nl_max;
   int nl_mode;  // 0 build, 1 use
   float nl_skin;
+  int symmetric;  // variable-h symmetric pairs
+  T* rate;        // SPHDensityRate output
 };
 
 // Scheme parameters of a pass from the ABI struct.
@@ -95,6 +97,23 @@ inline void set_params(PassArgs<T>& a, const sph_params* prm) {
   a.eps = prm->eps;
   for (int c = 0; c < 3; ++c) a.g[c] = prm->g[c];
   a.has_gravity = prm->has_gravity;
+  a.symmetric = prm->symmetric != 0;
+}
+
+// The pair test of the SoA passes: the destination's radius, or with
+// symmetric pairs the larger of the two radii (d2 < r_d^2 || d2 < r_s^2 is
+// d2 < max(r_d^2, r_s^2)).  r2f / r2d: the destination's radius squared.
+template <typename T, typename CT>
+__device__ __forceinline__ bool pair_test(const PassArgs<T>& a, const View<T>& S, int j, T xd,
+                                          T yd, T zd, float r2f, double r2d, CT& d2) {
+  if (a.symmetric) {
+    const double rs = (double)S.h[j] * a.rs;
+    const double r2s = rs * rs;
+    if (r2s > r2d) {
+      return Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, (float)r2s, r2s, d2);
+    }
+  }
+  return Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d, d2);
 }
 
 // Tait equation of state in FP64 (eos_funcs.pyx:105-116).

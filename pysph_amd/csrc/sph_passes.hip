@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
     for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
       for (int j = j0; j < j1; ++j) {
         CT d2;
-        if (!Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d, d2)) continue;
+        if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
         const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
         const CT inv = CT(1) / hab;
         const CT q = sqrt(d2) * inv;
@@ -63,6 +63,50 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
     a.p[i] = (T)p;
     a.cs[i] = (T)cs;
   }
+}
+
+// SPHDensityRate (density_funcs.pyx:89-158): sum_b m_b (v_a - v_b) .
+// grad_a W_ab with h_ab = (h_a + h_b)/2; non-LocalReal destinations get 0.
+template <typename T, typename CT>
+__global__ __launch_bounds__(kBlock) void density_rate_kernel(Grid g, PassArgs<T> a) {
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  const int i = bid * kBlock + threadIdx.x;
+  if (i >= a.dst.n) return;
+  if (a.dst.tag != nullptr && a.dst.tag[i] != 0) {
+    a.rate[i] = T(0);
+    return;
+  }
+  const T xd = a.dst.x[i], yd = a.dst.y[i], zd = a.dst.z[i];
+  const T ud = a.dst.u[i], vd = a.dst.v[i], wd = a.dst.w[i];
+  const CT ha = (CT)a.dst.h[i];
+  const double r = (double)a.dst.h[i] * a.rs;
+  const double r2d = r * r;
+  const float r2f = (float)r2d;
+  int cx, cy, cz;
+  cell_xyz(a.dst.cell[i], g, cx, cy, cz);
+  const int ring = ring_of(g, a.disp);
+  const int dim = a.dim;
+  CT rate = CT(0);
+  for (int s = 0; s < a.nsrc; ++s) {
+    const View<T> S = a.src[s];
+    CT nr = CT(0);
+    for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
+      for (int j = j0; j < j1; ++j) {
+        CT d2;
+        if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
+        const CT hab = CT(0.5) * (ha + (CT)S.h[j]);
+        const CT inv = CT(1) / hab;
+        const CT rr = sqrt(d2);
+        const CT q = rr * inv;
+        const CT gf = grad_shape<CT>(q, rr, hab, inv) * h_dim_inv<CT>(inv, dim);
+        const CT rx = (CT)xd - (CT)S.x[j], ry = (CT)yd - (CT)S.y[j], rz = (CT)zd - (CT)S.z[j];
+        const CT vx = (CT)ud - (CT)S.u[j], vy = (CT)vd - (CT)S.v[j], vz = (CT)wd - (CT)S.w[j];
+        nr += (vx * rx + vy * ry + vz * rz) * gf * (CT)S.m[j];
+      }
+    });
+    rate += nr;
+  }
+  a.rate[i] = (T)(rate * (CT)a.fac1);
 }
 
 template <typename T>
@@ -120,7 +164,7 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
       for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
         for (int j = j0; j < j1; ++j) {
           CT d2;
-          if (!Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d, d2)) continue;
+          if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
           const CT hab = CT(0.5) * (ha + (CT)S.h[j]);
           const CT inv = CT(1) / hab;
           const CT rr = sqrt(d2);
@@ -306,7 +350,7 @@ static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params*
   const bool do_tait = p != nullptr && cs != nullptr;
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
-  if (packed_inputs(a, false)) {
+  if (packed_inputs(a, false) && !a.symmetric) {
     if (nl != nullptr) {
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
@@ -387,7 +431,7 @@ static int32_t momentum_impl(hipStream_t st, int mode, const sph_grid* g, const 
   }
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
-  if (packed_inputs(a, true)) {
+  if (packed_inputs(a, true) && !a.symmetric) {
     if (nl != nullptr) {
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
@@ -405,11 +449,45 @@ static int32_t momentum_impl(hipStream_t st, int mode, const sph_grid* g, const 
   return launch_status(who);
 }
 
+template <typename T>
+static int32_t density_rate_impl(hipStream_t st, const sph_grid* g, const sph_params* prm,
+                                 const sph_view* dst, int32_t nsrc, const sph_view* srcs,
+                                 double rmax, const float* disp, void* rate) {
+  PassArgs<T> a{};
+  SPH_TRY(fill_views<T>(a, dst, nsrc, srcs, "sph_density_rate"));
+  set_params(a, prm);
+  a.disp = disp;
+  a.rate = (T*)rate;
+  Grid G = make_grid(*g, rmax);
+  if (dst->n == 0) return SPH_OK;
+  const int nb = nblocks_for(a.dst.n);
+  if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
+    hipLaunchKernelGGL((density_rate_kernel<T, double>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  else
+    hipLaunchKernelGGL((density_rate_kernel<T, T>), dim3(nb), dim3(kBlock), 0, st, G, a);
+  return launch_status("sph_density_rate");
+}
+
 }  // namespace sph
 
 using namespace sph;
 
 extern "C" {
+
+int32_t sph_density_rate(void* stream, const sph_grid* g, const sph_params* prm,
+                         const sph_view* dst, int32_t nsrc, const sph_view* srcs, double rmax,
+                         const float* disp_dev, void* rate_dev) {
+  SPH_TRY(check_common(g, prm, dst, nsrc, srcs, true, "sph_density_rate"));
+  if (dst->n > 0 && rate_dev == nullptr) {
+    set_error("sph_density_rate: NULL output");
+    return SPH_EINVAL;
+  }
+  if (dst->dtype == SPH_F32)
+    return density_rate_impl<float>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,
+                                     rate_dev);
+  return density_rate_impl<double>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,
+                                   rate_dev);
+}
 
 int32_t sph_density(void* stream, const sph_grid* g, const sph_params* prm, const sph_view* dst,
                     int32_t nsrc, const sph_view* srcs, double rmax, const float* disp_dev,

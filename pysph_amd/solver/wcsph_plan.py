@@ -63,7 +63,8 @@ class WCSPHPlan:
             co=t.co if t else 1.0, ro=t.ro if t else 1000.0, gamma=t.gamma if t else 7.0,
             alpha=f.alpha, beta=f.beta, eta=f.eta,
             eps=xsph.funcs[0].eps if xsph else 0.0, g=g, has_gravity=grav is not None,
-            has_xsph=xsph is not None)
+            has_xsph=xsph is not None,
+            symmetric=bool(getattr(particles, "variable_h", False)))
         # sources of the fused momentum pass: union of momentum and XSPH
         # sources in particle-array order, with their roles
         srcs, flags = [], []

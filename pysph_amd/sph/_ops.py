@@ -20,7 +20,9 @@ DEFAULT_PAIR_PREC = _abi.SPH_PAIR_NATIVE
 
 def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, beta=1.0,
                 eta=0.1, eps=0.5, g=(0.0, 0.0, 0.0), has_gravity=False, has_xsph=False,
-                pair_prec=None):
+                pair_prec=None, symmetric=False):
+    """sph_params; `symmetric` selects the variable-h pair sets of
+    VarHNbrParticleLocator (nnps.pyx:654-749)."""
     p = _abi.SphParams()
     p.dim = int(dim)
     p.pair_prec = DEFAULT_PAIR_PREC if pair_prec is None else int(pair_prec)
@@ -31,7 +33,18 @@ def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, 
         p.g[i] = float(g[i])
     p.has_gravity = 1 if has_gravity else 0
     p.has_xsph = 1 if has_xsph else 0
+    p.symmetric = 1 if symmetric else 0
     return p
+
+
+def calc_params(calc, **kw):
+    """make_params for a calc: its kernel's dimension and radius, and the
+    symmetric pair sets when its Particles were made with variable_h."""
+    sym = getattr(calc, "symmetric", None)
+    if sym is None:
+        sym = getattr(getattr(calc, "particles", None), "variable_h", False)
+    return make_params(calc.kernel.dim, radius_scale=calc.kernel.radius(), symmetric=bool(sym),
+                       **kw)
 
 
 def views(arrays, override=None):
@@ -99,6 +112,16 @@ def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, 
     vs = views(srcs, override)
     _abi.call("sph_density", stream, cm.grid, prm, vd, len(srcs), vs,
               search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs), nl)
+
+
+def density_rate(cm, prm, dst, srcs, rate, disp=None, override=None):
+    """SPHDensityRate over srcs into rate (density_funcs.pyx:89-158)."""
+    check_same_dtype([dst] + list(srcs))
+    stream = require_gpu(rate)
+    vd = views([dst], override)
+    vs = views(srcs, override)
+    _abi.call("sph_density_rate", stream, cm.grid, prm, vd, len(srcs), vs,
+              search_radius(cm, prm.radius_scale), ptr(disp), ptr(rate))
 
 
 def tait(prm, dst, p, cs, rho=None):

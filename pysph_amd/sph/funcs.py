@@ -1,6 +1,7 @@
 """The WCSPH operators of the hot path.
 
   SPHRho            sph/funcs/density_funcs.pyx:15-84
+  SPHDensityRate    sph/funcs/density_funcs.pyx:86-158
   TaitEquation      sph/funcs/eos_funcs.pyx:58-116
   MomentumEquation  sph/funcs/pressure_funcs.pyx:106-247
   XSPHCorrection    sph/funcs/xsph_funcs.pyx:8-88
@@ -32,8 +33,27 @@ class SPHRho(CSPHFunctionParticle):
     @classmethod
     def calc_eval(cls, calc, outputs):
         cm = grid_of(calc)
-        prm = _ops.make_params(calc.kernel.dim, radius_scale=calc.kernel.radius())
+        prm = _ops.calc_params(calc)
         _ops.density(cm, prm, calc.dest, calc.sources, outputs[0])
+
+
+class SPHDensityRate(SPHFunctionParticle):
+    """Continuity equation: drho_a/dt = sum_b m_b (v_a - v_b) . grad_a W_ab."""
+
+    def __init__(self, source, dest, setup_arrays=True, **kwargs):
+        SPHFunctionParticle.__init__(self, source, dest, setup_arrays=True, **kwargs)
+        self.id = "densityrate"
+        self.tag = "density"
+        self.num_outputs = 1
+
+    def set_src_dst_reads(self):
+        self.src_reads = ["x", "y", "z", "h", "m", "u", "v", "w"]
+        self.dst_reads = ["x", "y", "z", "h", "tag", "u", "v", "w"]
+
+    @classmethod
+    def calc_eval(cls, calc, outputs):
+        cm = grid_of(calc)
+        _ops.density_rate(cm, _ops.calc_params(calc), calc.dest, calc.sources, outputs[0])
 
 
 class TaitEquation(SPHFunction):
@@ -88,8 +108,7 @@ class MomentumEquation(SPHFunctionParticle):
     def calc_eval(cls, calc, outputs):
         f = calc.funcs[0]
         cm = grid_of(calc)
-        prm = _ops.make_params(calc.kernel.dim, radius_scale=calc.kernel.radius(), alpha=f.alpha,
-                               beta=f.beta, eta=f.eta)
+        prm = _ops.calc_params(calc, alpha=f.alpha, beta=f.beta, eta=f.eta)
         acc = list(outputs) + [None] * (3 - len(outputs))
         _ops.momentum_xsph(cm, prm, calc.dest, calc.sources,
                            [_abi.SPH_SRC_MOMENTUM] * len(calc.sources), acc=acc)
@@ -113,7 +132,7 @@ class XSPHCorrection(CSPHFunctionParticle):
     def calc_eval(cls, calc, outputs):
         f = calc.funcs[0]
         cm = grid_of(calc)
-        prm = _ops.make_params(calc.kernel.dim, radius_scale=calc.kernel.radius(), eps=f.eps)
+        prm = _ops.calc_params(calc, eps=f.eps)
         dv = list(outputs) + [None] * (3 - len(outputs))
         _ops.momentum_xsph(cm, prm, calc.dest, calc.sources,
                            [_abi.SPH_SRC_XSPH] * len(calc.sources), dv=dv)

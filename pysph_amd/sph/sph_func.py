@@ -106,6 +106,8 @@ class _SingleCalc:
         self.dim = kernel.dim
         loc = func.nbr_locator
         self.cell_manager = loc.cell_manager if loc is not None else None
+        # the pair sets follow the function's locator
+        self.symmetric = bool(getattr(loc, "symmetric", False))
 
 
 class SPHFunctionParticle(SPHFunction):
