@@ -305,6 +305,56 @@ class CellManager:
     def get_number_of_particles(self):
         return sum(pa.get_number_of_particles() for pa in self.arrays_to_bin)
 
+    def get_particle_representation(self, fname, fmt="pickle"):
+        """Write the particles of every cell (cell.pyx:1505-1514, :702-731).
+
+        fmt "pickle": the reference's dict {(i, j, k): {'cid', 'centroid',
+        'positions': {array name: {'x', 'y', 'z', 'idx'}}}} of plain Python
+        types, centroid = (cid + 0.5) * cell_size.  fmt "npz": per array
+        `<name>_x/_y/_z/_idx` (float64 / int64, user order) and `<name>_cid`
+        (int32 [n,3], floor(coordinate / cell_size)), plus cell_size.
+        Returns the file name written."""
+        import pickle
+
+        cs = self.cell_size
+        per = {}
+        for pa in self.arrays_to_bin:
+            x, y, z = (pa.get(c).astype(np.float64) for c in ("x", "y", "z"))
+            idx = pa.get("idx").astype(np.int64) if pa.has_array("idx") else \
+                np.arange(len(x), dtype=np.int64)
+            cid = np.stack([np.floor(a / cs) for a in (x, y, z)], 1).astype(np.int32)
+            per[pa.name] = (x, y, z, idx, cid)
+        if fmt == "npz":
+            out = {"cell_size": cs}
+            for name, (x, y, z, idx, cid) in per.items():
+                out.update({name + "_x": x, name + "_y": y, name + "_z": z, name + "_idx": idx,
+                            name + "_cid": cid})
+            if not fname.endswith(".npz"):
+                fname = fname + ".npz"
+            np.savez(fname, **out)
+            return fname
+        if fmt != "pickle":
+            raise ValueError("fmt must be 'pickle' or 'npz'")
+        ret = {}
+        for name, (x, y, z, idx, cid) in per.items():
+            for k in range(len(x)):
+                key = tuple(int(c) for c in cid[k])
+                cell = ret.get(key)
+                if cell is None:
+                    cell = {"cid": key,
+                            "centroid": tuple((c + 0.5) * cs for c in key),
+                            "positions": {p.name: {"x": [], "y": [], "z": [], "idx": []}
+                                          for p in self.arrays_to_bin}}
+                    ret[key] = cell
+                d = cell["positions"][name]
+                d["x"].append(float(x[k]))
+                d["y"].append(float(y[k]))
+                d["z"].append(float(z[k]))
+                d["idx"].append(int(idx[k]))
+        with open(fname, "wb") as f:
+            pickle.dump(ret, f, protocol=2)
+        return fname
+
     def cells_of(self, pa):
         """Host view of the binning of pa: dict IntPoint -> user indices (the
         reference's cells_dict content for one array)."""

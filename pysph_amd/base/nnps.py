@@ -38,11 +38,24 @@ def _csr_user_order(pa_dst, off_slot, nbr, n):
     ucounts[order] = counts
     uoff = np.zeros(n + 1, dtype=np.int64)
     np.cumsum(ucounts, out=uoff[1:])
+    # entry e of slot row s moves to uoff[order[s]] + (e - off_slot[s])
+    row = np.repeat(np.arange(n, dtype=np.int64), counts)
+    dest = uoff[order[row]] + (np.arange(len(nbr), dtype=np.int64) - off_slot[row])
     out = np.empty_like(nbr)
-    for s in range(n):
-        u = order[s]
-        out[uoff[u]:uoff[u + 1]] = nbr[off_slot[s]:off_slot[s + 1]]
+    out[dest] = nbr
     return uoff, out
+
+
+def _self_last(off, nb):
+    """Move each row's own index to the row's last slot (nnps.pyx:612-618);
+    rows hold an index at most once."""
+    n = len(off) - 1
+    row = np.repeat(np.arange(n, dtype=np.int64), np.diff(off))
+    pos = np.nonzero(nb == row)[0]
+    last = off[row[pos] + 1] - 1
+    nb[pos] = nb[last]
+    nb[last] = row[pos]
+    return nb
 
 
 def neighbor_csr(cell_manager, src, dst, radius_scale, symmetric=False):
@@ -152,12 +165,7 @@ class FixedDestNbrParticleLocator(NbrParticleLocatorBase):
                                    self.radius_scale, symmetric=self.symmetric)
             if self.source is self.dest:
                 # keep the particle itself in the last slot (nnps.pyx:612-618)
-                for i in range(len(off) - 1):
-                    row = nb[off[i]:off[i + 1]]
-                    hit = np.nonzero(row == i)[0]
-                    if len(hit):
-                        k = hit[0]
-                        row[k], row[-1] = row[-1], row[k]
+                nb = _self_last(off, nb)
             self._off, self._nbr = off, nb
             self.is_dirty = False
         return 0
