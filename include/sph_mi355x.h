@@ -80,6 +80,15 @@ typedef struct sph_grid {
     double cell_size;
     int32_t lo[3];
     int32_t dims[3]; /* 1 for unused axes */
+    /* Periodic axes (PeriodicDomain, cell.pyx:1568-1596): period[a] > 0
+     * makes axis a periodic over [pmin[a], pmin[a] + period[a]); 0 = not
+     * periodic.  The neighbour searches then also visit the destination's
+     * periodic images and see a source at x_j + s*period (s = -1, 0, 1), the
+     * coordinate of the reference's ghost copy (cell.pyx:1277-1376), added in
+     * FP64.  A periodic call needs period > 2 * the search radius on that
+     * axis.  The packed fast path and neighbour lists are not used then. */
+    double period[3];
+    double pmin[3];
 } sph_grid;
 
 /* Parameters of the WCSPH operations.
@@ -143,6 +152,17 @@ int32_t sph_bin(void *stream, const sph_view *a, const sph_grid *g,
 int32_t sph_permute(void *stream, int64_t n, const int32_t *perm_dev,
                     int32_t nfields, const void *const *src_dev_host,
                     void *const *dst_dev_host, const int32_t *elem_bytes_host);
+
+/* Wrap coordinates into the periodic domain of `g`: per periodic axis a,
+ * a particle with pos[a] >= pmin + period moves by -period, one with
+ * pos[a] < pmin by +period (the binning point rule of cell.pyx:392-398,
+ * applied to the stored coordinate).  Each of the nextra arrays of
+ * extra_dev_host[3*k + a] (e.g. the integrator's copies of x, y, z) moves
+ * by the same amount as pos[a].  Arrays of dtype, n elements each; NULL
+ * entries for non-periodic axes are allowed. */
+int32_t sph_periodic_wrap(void *stream, const sph_grid *g, int32_t dtype, int64_t n,
+                          void *const *pos_dev_host, int32_t nextra,
+                          void *const *extra_dev_host);
 
 /* Write the packed copies pos4[i] = (x,y,z,h) and vel4[i] = (u,v,w,m) of an
  * FP32 array (float4 each; either may be NULL). */

@@ -191,6 +191,76 @@ def nnps_csr(dst, src=None, cell_size=None, radius_scale=2.0, varh=False):
     return off, nb[:total]
 
 
+def ghost_cells(arrs, period, pmin, cell_size, dist):
+    """CellManager.create_ghost_cells (cell.pyx:1277-1376) on the cells of
+    `arrs` (cell = floor(x / cell_size) per axis, cell.pyx:42-82): a cell
+    whose lower face lies at or below the domain's lower face, or within
+    `dist` of it, is copied one period up; one whose upper face lies at or
+    above the upper face, or within `dist` of it, one period down.  The
+    reference does this along x only; along several periodic axes the
+    copies combine (edges and corners).  Returns (real cell ids, dict ghost
+    cell id -> list of (shift vector, source cell id))."""
+    cells = set()
+    for a in arrs:
+        for c in zip(*[np.floor(np.asarray(a[k]) / cell_size).astype(np.int64)
+                       for k in ("x", "y", "z")]):
+            cells.add(tuple(int(v) for v in c))
+    ghosts = {}
+    for cid in sorted(cells):
+        opts = []
+        for d in range(3):
+            o = [0]
+            if period[d] > 0:
+                centre = (cid[d] + 0.5) * cell_size
+                lower, upper = centre - 0.5 * cell_size, centre + 0.5 * cell_size
+                pmax = pmin[d] + period[d]
+                if lower <= pmin[d] or (lower - pmin[d]) < dist:
+                    o.append(1)
+                if upper >= pmax or (pmax - upper) < dist:
+                    o.append(-1)
+            opts.append(o)
+        for sx in opts[0]:
+            for sy in opts[1]:
+                for sz in opts[2]:
+                    if sx == 0 and sy == 0 and sz == 0:
+                        continue
+                    sh = (sx, sy, sz)
+                    centre = [(cid[d] + 0.5) * cell_size + sh[d] * period[d] for d in range(3)]
+                    gid = tuple(int(np.floor(c / cell_size)) for c in centre)
+                    ghosts.setdefault(gid, []).append((sh, cid))
+    return cells, ghosts
+
+
+def with_ghosts(arr, period, pmin, cell_size, dist):
+    """`arr` extended by the ghost copies of its particles (Cell.copy,
+    cell.pyx:600-660: the copied particles' coordinates + shift * period,
+    added in FP64).  Returns (extended array, origin index of each entry)."""
+    n = len(arr["x"])
+    cid = np.stack([np.floor(np.asarray(arr[k]) / cell_size).astype(np.int64)
+                    for k in ("x", "y", "z")], 1)
+    _, ghosts = ghost_cells([arr], period, pmin, cell_size, dist)
+    by_cell = {}
+    for i in range(n):
+        by_cell.setdefault(tuple(int(v) for v in cid[i]), []).append(i)
+    idx, shifts = [np.arange(n)], [np.zeros((n, 3))]
+    for copies in ghosts.values():
+        for sh, src_cell in copies:
+            members = by_cell.get(src_cell, [])
+            if members:
+                idx.append(np.asarray(members))
+                shifts.append(np.tile(np.asarray(sh, dtype=np.float64) *
+                                      np.asarray(period, dtype=np.float64), (len(members), 1)))
+    origin = np.concatenate(idx)
+    sh = np.concatenate(shifts)
+    out = {}
+    for f in _FIELDS:
+        out[f] = np.ascontiguousarray(arr[f][origin])
+    for d, k in enumerate(("x", "y", "z")):
+        out[k] = out[k] + sh[:, d]
+    out["type"] = arr.get("type", 0)
+    return out, origin
+
+
 def point_query(src, cell_size, pnt, radius, exclude_index=-1):
     bs = _Bound(src)
     out = np.zeros(bs.s.n + 1, dtype=np.int64)

@@ -38,7 +38,8 @@ class SphView(ctypes.Structure):
 
 class SphGrid(ctypes.Structure):
     _fields_ = [("cell_size", ctypes.c_double), ("lo", ctypes.c_int32 * 3),
-                ("dims", ctypes.c_int32 * 3)]
+                ("dims", ctypes.c_int32 * 3), ("period", ctypes.c_double * 3),
+                ("pmin", ctypes.c_double * 3)]
 
 
 class SphParams(ctypes.Structure):
@@ -148,6 +149,8 @@ _SIGS = {
                                           ctypes.c_int32, _P(SphView), ctypes.c_double, _vp,
                                           _vp]),
     "sph_tait": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp, _vp]),
+    "sph_periodic_wrap": (ctypes.c_int32, [_vp, _P(SphGrid), ctypes.c_int32, ctypes.c_int64,
+                                           _P(_vp), ctypes.c_int32, _P(_vp)]),
     "sph_momentum_xsph": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                            ctypes.c_int32, _P(SphView), _P(ctypes.c_uint32),
                                            ctypes.c_double, _vp, _P(_vp), _P(_vp),

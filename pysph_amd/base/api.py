@@ -1,7 +1,7 @@
 """Public names of the base layer (reference: base/api.py)."""
 from .carray import DoubleArray, FloatArray, IntArray, LongArray  # noqa: F401
-from .cell import (CellManager, construct_immediate_neighbor_list, find_cell_id,  # noqa: F401
-                   py_find_cell_id, py_real_to_int, real_to_int)
+from .cell import (CellManager, PeriodicDomain, construct_immediate_neighbor_list,  # noqa: F401
+                   find_cell_id, py_find_cell_id, py_real_to_int, real_to_int)
 from .kernels import CubicSplineKernel, DummyKernel, KernelBase  # noqa: F401
 from .nnps import (FixedDestNbrParticleLocator, NbrParticleLocatorBase,  # noqa: F401
                    NeighborLocatorType, NNPSManager, VarHNbrParticleLocator, neighbor_csr)

@@ -17,7 +17,7 @@ import numpy as np
 import torch
 
 from .. import _abi
-from .._device import require_gpu
+from .._device import dtype_code, require_gpu
 from .point import IntPoint
 
 
@@ -70,13 +70,39 @@ def grid_for_domain(lo, hi, cell_size, slack=2):
     return g
 
 
+class PeriodicDomain:
+    """Limits of a periodic box (cell.pyx:1568-1596).  An axis whose limits
+    stay at the default +-1000 has translate 0 and is not periodic."""
+
+    def __init__(self, xmin=-1000, xmax=1000, ymin=-1000, ymax=1000, zmin=-1000, zmax=1000):
+        self.xmin, self.xmax = float(xmin), float(xmax)
+        self.ymin, self.ymax = float(ymin), float(ymax)
+        self.zmin, self.zmax = float(zmin), float(zmax)
+        self.xtranslate = 0.0 if (self.xmax - self.xmin) == 2000 else self.xmax - self.xmin
+        self.ytranslate = 0.0 if (self.ymax - self.ymin) == 2000 else self.ymax - self.ymin
+        self.ztranslate = 0.0 if (self.zmax - self.zmin) == 2000 else self.zmax - self.zmin
+
+    def periods(self):
+        """((period per axis, 0 = not periodic), (lower face per axis))."""
+        return ((self.xtranslate, self.ytranslate, self.ztranslate),
+                (self.xmin, self.ymin, self.zmin))
+
+
 class CellManager:
-    """Bins every array of `arrays_to_bin` on one shared grid."""
+    """Bins every array of `arrays_to_bin` on one shared grid.
+
+    With a periodic_domain the stored coordinates are wrapped into the box
+    before each binning (x >= max moves by -L, x < min by +L, the
+    reference's binning-point rule of cell.pyx:392-398 applied to the
+    coordinate, and to the integrator's `_x_0` copies by the same amount),
+    the jump check is off as in the reference, and the neighbour passes see
+    sources across each periodic face at x_j +- L, which is where the
+    reference's ghost cells (cell.pyx:1277-1376) put their copies; no ghost
+    particles are created."""
 
     def __init__(self, arrays_to_bin=(), min_cell_size=-1.0, max_cell_size=0,
                  periodic_domain=None, initialize=True, max_radius_scale=2.0):
-        if periodic_domain is not None:
-            raise NotImplementedError("periodic domains are not supported")
+        self.periodic_domain = periodic_domain
         self.arrays_to_bin = list(arrays_to_bin)
         self.min_cell_size = min_cell_size
         self.max_cell_size = max_cell_size
@@ -190,7 +216,45 @@ class CellManager:
         return lo, hi
 
     def make_grid(self, lo, hi):
-        return grid_for_domain(lo, hi, self.cell_size, self.grid_slack)
+        g = grid_for_domain(lo, hi, self.cell_size, self.grid_slack)
+        self._set_periodic(g)
+        return g
+
+    def _set_periodic(self, g):
+        if self.periodic_domain is None:
+            return g
+        per, pmin = self.periodic_domain.periods()
+        for d in range(3):
+            g.period[d] = float(per[d])
+            g.pmin[d] = float(pmin[d])
+        return g
+
+    def _check_period(self):
+        per, _ = self.periodic_domain.periods()
+        r = self.max_radius_scale * self.max_h
+        for d in range(3):
+            if per[d] > 0 and not per[d] > 2.0 * r:
+                raise ValueError("periodic axis %d: the period %g must exceed twice the "
+                                 "search radius %g" % (d, per[d], r))
+
+    def wrap_periodic(self, pa):
+        """Wrap pa's coordinates (and its `_x_0`-style integrator copies)
+        into the periodic box on the device."""
+        if self.periodic_domain is None or pa.get_number_of_particles() == 0:
+            return
+        g = self._set_periodic(_abi.SphGrid())
+        g.cell_size = self.cell_size or 1.0
+        for d in range(3):
+            g.dims[d] = 1
+        pos = [pa.get_carray(c) for c in ("x", "y", "z")]
+        extra = []
+        if all(pa.has_array("_%s_0" % c) for c in ("x", "y", "z")):
+            extra.append([pa.get_carray("_%s_0" % c) for c in ("x", "y", "z")])
+        stream = require_gpu(pos[0])
+        _abi.call("sph_periodic_wrap", stream, g, dtype_code(pa.dtype),
+                  pa.get_number_of_particles(), _abi.ptr_array([t.data_ptr() for t in pos]),
+                  len(extra), _abi.ptr_array([t.data_ptr() for e in extra for t in e])
+                  if extra else None)
 
     @staticmethod
     def grid_key(g):
@@ -213,6 +277,10 @@ class CellManager:
         self.update_status()
         if not self.is_dirty and self.grid is not None:
             return 0
+        if self.periodic_domain is not None:
+            for pa in self.arrays_to_bin:
+                if pa.is_dirty or pa._grid_key is None:
+                    self.wrap_periodic(pa)
         if self.fixed_domain is not None:
             lo, hi = self.fixed_domain
         else:
@@ -228,6 +296,8 @@ class CellManager:
                 for d in range(3))
             if covers:
                 g = old
+        if self.periodic_domain is not None:
+            self._check_period()
         key = self.grid_key(g)
         for pa in self.arrays_to_bin:
             if pa.is_dirty or pa._grid_key != key:
@@ -258,8 +328,10 @@ class CellManager:
         perm = torch.empty(n, dtype=torch.int32, device=dev)
         cell = torch.empty(n, dtype=torch.int32, device=dev)
         ws = self._workspace(n, ncells, dev)
-        prev = pa._cell if (self.check_jumps and pa._grid_key == key and
-                            pa._cell is not None and pa._cell.numel() == n) else None
+        # no jump check across a periodic wrap (cell.pyx:404-407)
+        prev = pa._cell if (self.check_jumps and self.periodic_domain is None and
+                            pa._grid_key == key and pa._cell is not None and
+                            pa._cell.numel() == n) else None
         jumps = None
         if prev is not None:
             if self.check_jumps == "deferred":

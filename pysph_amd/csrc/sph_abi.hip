@@ -49,6 +49,17 @@ int32_t check_view(const sph_view* a, const char* who, bool need_vel, bool need_
   return SPH_OK;
 }
 
+int32_t check_period(const sph_grid* g, double rmax, const char* who) {
+  for (int d = 0; d < 3; ++d) {
+    if (g->period[d] > 0.0 && !(g->period[d] > 2.0 * rmax)) {
+      set_error("%s: periodic axis %d: period %g must exceed twice the search radius %g", who,
+                d, g->period[d], rmax);
+      return SPH_EINVAL;
+    }
+  }
+  return SPH_OK;
+}
+
 int32_t check_grid(const sph_grid* g, const char* who) {
   if (g == nullptr) {
     set_error("%s: NULL grid", who);
@@ -61,6 +72,12 @@ int32_t check_grid(const sph_grid* g, const char* who) {
   for (int d = 0; d < 3; ++d) {
     if (g->dims[d] < 1) {
       set_error("%s: grid dims must be >= 1", who);
+      return SPH_EINVAL;
+    }
+  }
+  for (int d = 0; d < 3; ++d) {
+    if (g->period[d] < 0.0 || !isfinite(g->period[d]) || !isfinite(g->pmin[d])) {
+      set_error("%s: periodic axis %d: period must be finite and >= 0", who, d);
       return SPH_EINVAL;
     }
   }
@@ -85,6 +102,12 @@ Grid make_grid(const sph_grid& g, double rmax) {
   int r = (int)ceil(rmax / g.cell_size);
   G.ring0 = r < 1 ? 1 : r;
   G.cell_base = 0;
+  G.periodic = 0;
+  for (int d = 0; d < 3; ++d) {
+    G.per[d] = g.period[d] > 0.0 ? g.period[d] : 0.0;
+    G.pmin[d] = g.pmin[d];
+    if (G.per[d] > 0.0) G.periodic = 1;
+  }
   return G;
 }
 

@@ -278,6 +278,10 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
       set_error("sph_batch_create: scene %d: rmax, radius_scale and dt must be positive", s);
       return SPH_EINVAL;
     }
+    if (S.grid.period[0] > 0 || S.grid.period[1] > 0 || S.grid.period[2] > 0) {
+      set_error("sph_batch_create: scene %d: periodic grids are not supported", s);
+      return SPH_EUNSUPPORTED;
+    }
     if (S.prm.symmetric) {
       set_error("sph_batch_create: scene %d: variable-h symmetric pairs are not supported", s);
       return SPH_EUNSUPPORTED;

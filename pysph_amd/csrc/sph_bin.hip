@@ -336,7 +336,6 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   const T xd = a.dst.x[i], yd = a.dst.y[i], zd = a.dst.z[i];
   const double r = (double)a.dst.h[i] * a.rs;  // eff_radius (nnps.pyx:552)
   const double r2d = r * r;
-  const float r2f = (float)r2d;
   int cx, cy, cz;
   cell_xyz(a.dst.cell[i], g, cx, cy, cz);
   // ring = ceil(r/cs) for this destination (cell.pyx:1127-1129); the
@@ -347,20 +346,30 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   if (ring < 1) ring = 1;
   int64_t k = FILL ? a.off[i] : 0;
   const View<T>& S = a.src;
-  for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
-    for (int j = j0; j < j1; ++j) {
-      bool hit = pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d);
-      if (!hit && a.rmax_sym > 0) {
-        // the reverse locator's test: d2 < (h_s * rs)^2 (nnps.pyx:735-747)
-        const double rsj = (double)S.h[j] * a.rs;
-        const double r2s = rsj * rsj;
-        hit = pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, (float)r2s, r2s);
+  for_each_image(g, (double)xd, (double)yd, (double)zd, cx, cy, cz, ring * g.cs,
+                 [&](double ox, double oy, double oz, int icx, int icy, int icz, bool image) {
+    for_each_row(g, S.cstart, icx, icy, icz, image ? ring + 1 : ring, [&](int j0, int j1) {
+      for (int j = j0; j < j1; ++j) {
+        double r2 = r2d;
+        if (a.rmax_sym > 0) {
+          // the reverse locator's test: d2 < (h_s * rs)^2 (nnps.pyx:735-747)
+          const double rsj = (double)S.h[j] * a.rs;
+          if (rsj * rsj > r2) r2 = rsj * rsj;
+        }
+        bool hit;
+        if (!image) {
+          hit = pair_within<T>(S.x[j], S.y[j], S.z[j], xd, yd, zd, (float)r2, r2);
+        } else {
+          // a ghost copy's coordinate, x_j + s*L, in FP64 (cell.pyx:1277-1376)
+          hit = exact_within((double)S.x[j] + ox, (double)S.y[j] + oy, (double)S.z[j] + oz,
+                             (double)xd, (double)yd, (double)zd, r2);
+        }
+        if (hit) {
+          if (FILL) a.out[k] = a.ids[j];
+          ++k;
+        }
       }
-      if (hit) {
-        if (FILL) a.out[k] = a.ids[j];
-        ++k;
-      }
-    }
+    });
   });
   if (!FILL) {
     a.counts[i] = k;
@@ -583,6 +592,7 @@ static int32_t nbr_counts(void* stream, const sph_grid* g, const sph_view* dst,
   }
   hipStream_t st = as_stream(stream);
   Grid G = make_grid(*g, 0.0);
+  if (rmax_sym > 0) SPH_TRY(check_period(g, rmax_sym, who));
   const int nb = nblocks_for(dst->n);
   if (dst->dtype == SPH_F32) {
     NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, nullptr,

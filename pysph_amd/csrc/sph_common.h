@@ -34,6 +34,9 @@ struct Grid {
   double rmax;   // largest search radius of the call (radius_scale * max h)
   int ring0;     // ceil(rmax / cs)
   int cell_base; // cell ids stored per particle are cell_base + local id
+  int periodic;  // any periodic axis
+  double per[3];  // period per axis, 0 = not periodic
+  double pmin[3]; // lower face of the periodic domain
 };
 
 __device__ __forceinline__ int clampi(int v, int lo, int hi) {
@@ -139,6 +142,49 @@ __device__ __forceinline__ void for_each_row(const Grid& g, const int* __restric
       f(j0, j1);
     }
   }
+}
+
+// Periodic images of a destination at p in cell (cx,cy,cz): f(ox,oy,oz,
+// icx,icy,icz,image) runs for the destination itself (offset 0) and, along
+// periodic axes, for every shift s in {-1,0,1}^3 whose image point p - s*L
+// comes within `reach` of the domain: sources are then seen at x_j + s*L,
+// the ghost copy's coordinate (cell.pyx:1277-1376).  With period > 2*reach
+// a source matches at most one image.  Image cells come from the image
+// point in FP64; unshifted axes keep the destination's own cell.
+template <class F>
+__device__ __forceinline__ void for_each_image(const Grid& g, double px, double py, double pz,
+                                               int cx, int cy, int cz, double reach, F&& f) {
+  f(0.0, 0.0, 0.0, cx, cy, cz, false);
+  if (!g.periodic) return;
+  const double p[3] = {px, py, pz};
+  const int c0[3] = {cx, cy, cz};
+  int lo[3], hi[3];
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = 0;
+    hi[a] = 0;
+    if (g.per[a] > 0.0) {
+      if (p[a] + reach >= g.pmin[a] + g.per[a]) hi[a] = 1;
+      if (p[a] - reach < g.pmin[a]) lo[a] = -1;
+    }
+  }
+  for (int sx = lo[0]; sx <= hi[0]; ++sx)
+    for (int sy = lo[1]; sy <= hi[1]; ++sy)
+      for (int sz = lo[2]; sz <= hi[2]; ++sz) {
+        if (sx == 0 && sy == 0 && sz == 0) continue;
+        const int sh[3] = {sx, sy, sz};
+        double o[3];
+        int c[3];
+        for (int a = 0; a < 3; ++a) {
+          o[a] = sh[a] * g.per[a];
+          if (sh[a] == 0) {
+            c[a] = c0[a];
+          } else {
+            const double k = floor((p[a] - o[a]) / g.cs) - (double)g.lo[a];
+            c[a] = k < 0.0 ? 0 : (k > (double)(g.dims[a] - 1) ? g.dims[a] - 1 : (int)k);
+          }
+        }
+        f(o[0], o[1], o[2], c[0], c[1], c[2], true);
+      }
 }
 
 template <typename T>

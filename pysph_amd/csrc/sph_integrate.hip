@@ -183,7 +183,85 @@ __global__ void cons_final(const double* __restrict__ part, int nb, double* __re
 
 using namespace sph;
 
+namespace sph {
+
+constexpr int kMaxWrapExtra = 6;
+
+template <typename T>
+struct WrapArgs {
+  T* pos[3];
+  T* extra[kMaxWrapExtra][3];
+  int nextra;
+  int n;
+  T lo[3], hi[3], per[3];
+};
+
+// Periodic wrap of the stored coordinates (the binning point rule of
+// cell.pyx:392-398): one shift per axis, from the position, applied to the
+// position and to every extra copy.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void wrap_kernel(WrapArgs<T> w) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= w.n) return;
+  for (int a = 0; a < 3; ++a) {
+    if (w.pos[a] == nullptr) continue;
+    const T x = w.pos[a][i];
+    T sh = T(0);
+    if (x >= w.hi[a]) sh = -w.per[a];
+    else if (x < w.lo[a]) sh = w.per[a];
+    if (sh == T(0)) continue;
+    w.pos[a][i] = x + sh;
+    for (int k = 0; k < w.nextra; ++k)
+      if (w.extra[k][a] != nullptr) w.extra[k][a][i] = w.extra[k][a][i] + sh;
+  }
+}
+
+template <typename T>
+static int32_t wrap_impl(hipStream_t st, const sph_grid* g, int64_t n, void* const* pos,
+                         int32_t nextra, void* const* extra) {
+  WrapArgs<T> w{};
+  w.n = (int)n;
+  w.nextra = nextra;
+  for (int a = 0; a < 3; ++a) {
+    const bool per = g->period[a] > 0.0;
+    w.pos[a] = per ? (T*)pos[a] : nullptr;
+    if (per && w.pos[a] == nullptr) {
+      set_error("sph_periodic_wrap: periodic axis %d has no coordinate array", a);
+      return SPH_EINVAL;
+    }
+    w.lo[a] = (T)g->pmin[a];
+    w.hi[a] = (T)(g->pmin[a] + g->period[a]);
+    w.per[a] = (T)g->period[a];
+    for (int k = 0; k < nextra; ++k) w.extra[k][a] = per ? (T*)extra[3 * k + a] : nullptr;
+  }
+  if (n == 0) return SPH_OK;
+  hipLaunchKernelGGL(wrap_kernel<T>, dim3(nblocks_for(n)), dim3(kBlock), 0, st, w);
+  return launch_status("sph_periodic_wrap");
+}
+
+}  // namespace sph
+
+using namespace sph;
+
 extern "C" {
+
+int32_t sph_periodic_wrap(void* stream, const sph_grid* g, int32_t dtype, int64_t n,
+                          void* const* pos_dev_host, int32_t nextra,
+                          void* const* extra_dev_host) {
+  if (g == nullptr || n < 0 || n > (int64_t)0x7fffffff || nextra < 0 ||
+      nextra > kMaxWrapExtra || (n > 0 && pos_dev_host == nullptr) ||
+      (nextra > 0 && extra_dev_host == nullptr)) {
+    set_error("sph_periodic_wrap: bad arguments (at most %d extra arrays)", kMaxWrapExtra);
+    return SPH_EINVAL;
+  }
+  SPH_TRY(check_grid(g, "sph_periodic_wrap"));
+  if (dtype == SPH_F32)
+    return wrap_impl<float>(as_stream(stream), g, n, pos_dev_host, nextra, extra_dev_host);
+  if (dtype == SPH_F64)
+    return wrap_impl<double>(as_stream(stream), g, n, pos_dev_host, nextra, extra_dev_host);
+  set_error("sph_periodic_wrap: dtype unsupported");
+  return SPH_EUNSUPPORTED;
+}
 
 int32_t sph_update(void* stream, int32_t dtype, int64_t n, int32_t mode, double s, int32_t nops,
                    const sph_update_op* ops_host, const int64_t* tag_dev) {

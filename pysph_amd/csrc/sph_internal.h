@@ -29,6 +29,8 @@ inline int nblocks_for(int64_t n, int per_block = kBlock) {
 int32_t check_view(const sph_view* a, const char* who, bool need_vel, bool need_state,
                    bool need_bins);
 int32_t check_grid(const sph_grid* g, const char* who);
+// A periodic axis must be longer than twice the search radius.
+int32_t check_period(const sph_grid* g, double rmax, const char* who);
 Grid make_grid(const sph_grid& g, double rmax);
 
 // Grid of one scene of a multi-scene binning.

@@ -100,20 +100,52 @@ inline void set_params(PassArgs<T>& a, const sph_params* prm) {
   a.symmetric = prm->symmetric != 0;
 }
 
-// The pair test of the SoA passes: the destination's radius, or with
-// symmetric pairs the larger of the two radii (d2 < r_d^2 || d2 < r_s^2 is
-// d2 < max(r_d^2, r_s^2)).  r2f / r2d: the destination's radius squared.
+// Distance to a periodic image source at (sx,sy,sz), FP64 without
+// contraction like the test on a ghost copy (point.pxd:35-37).
+template <typename CT>
+__device__ __forceinline__ bool image_geom(double sx, double sy, double sz, double xd, double yd,
+                                           double zd, double r2d, CT& d2, CT& rx, CT& ry,
+                                           CT& rz) {
+#pragma clang fp contract(off)
+  const double ex = sx - xd, ey = sy - yd, ez = sz - zd;
+  const double dd = (ex * ex + ey * ey) + ez * ez;
+  if (!(dd < r2d)) return false;
+  d2 = (CT)dd;
+  rx = (CT)(xd - sx);
+  ry = (CT)(yd - sy);
+  rz = (CT)(zd - sz);
+  return true;
+}
+
+// The pair test and geometry of the SoA passes.  Cut-off: the
+// destination's radius, or with symmetric pairs the larger of the two radii
+// (d2 < r_d^2 || d2 < r_s^2 is d2 < max(r_d^2, r_s^2)); r2f / r2d: the
+// destination's radius squared.  (ox, oy, oz) shifts the source to a
+// periodic image: the shifted coordinate and the distance are then formed
+// in FP64 like a ghost copy's ((x_j + L) - x_i), whatever the storage type.
+// On acceptance d2 and r = x_d - x_s (in CT) are set.
 template <typename T, typename CT>
-__device__ __forceinline__ bool pair_test(const PassArgs<T>& a, const View<T>& S, int j, T xd,
-                                          T yd, T zd, float r2f, double r2d, CT& d2) {
+__device__ __forceinline__ bool pair_geom(const PassArgs<T>& a, const View<T>& S, int j, T xd,
+                                          T yd, T zd, float r2f, double r2d, bool image,
+                                          double ox, double oy, double oz, CT& d2, CT& rx,
+                                          CT& ry, CT& rz) {
   if (a.symmetric) {
     const double rs = (double)S.h[j] * a.rs;
     const double r2s = rs * rs;
     if (r2s > r2d) {
-      return Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, (float)r2s, r2s, d2);
+      r2d = r2s;
+      r2f = (float)r2s;
     }
   }
-  return Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d, d2);
+  if (!image) {
+    if (!Cutoff<T, CT>::test(S.x[j], S.y[j], S.z[j], xd, yd, zd, r2f, r2d, d2)) return false;
+    rx = (CT)xd - (CT)S.x[j];
+    ry = (CT)yd - (CT)S.y[j];
+    rz = (CT)zd - (CT)S.z[j];
+    return true;
+  }
+  return image_geom<CT>((double)S.x[j] + ox, (double)S.y[j] + oy, (double)S.z[j] + oz,
+                        (double)xd, (double)yd, (double)zd, r2d, d2, rx, ry, rz);
 }
 
 // Tait equation of state in FP64 (eos_funcs.pyx:105-116).

@@ -17,6 +17,29 @@
 
 namespace sph {
 
+// Every accepted source j of destination (xd,yd,zd) in cell (cx,cy,cz):
+// the stencil rows of the destination and of its periodic images (one ring
+// wider, their cells come from FP64 image points), each pair through
+// pair_geom; body(j, d2, rx, ry, rz).
+template <typename T, typename CT, class F>
+__device__ __forceinline__ void scan_pairs(const Grid& g, const PassArgs<T>& a, const View<T>& S,
+                                           T xd, T yd, T zd, int cx, int cy, int cz, int ring,
+                                           float r2f, double r2d, F&& body) {
+  for_each_image(g, (double)xd, (double)yd, (double)zd, cx, cy, cz, ring * g.cs,
+                 [&](double ox, double oy, double oz, int icx, int icy, int icz, bool image) {
+                   for_each_row(g, S.cstart, icx, icy, icz, image ? ring + 1 : ring,
+                                [&](int j0, int j1) {
+                                  for (int j = j0; j < j1; ++j) {
+                                    CT d2, rx, ry, rz;
+                                    if (!pair_geom<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, image,
+                                                          ox, oy, oz, d2, rx, ry, rz))
+                                      continue;
+                                    body(j, d2, rx, ry, rz);
+                                  }
+                                });
+                 });
+}
+
 template <typename T, typename CT, bool TAIT>
 __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) {
   const int bid = xcd_block(blockIdx.x, gridDim.x);
@@ -43,16 +66,13 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
   for (int s = 0; s < a.nsrc; ++s) {
     const View<T> S = a.src[s];
     CT nr = CT(0);
-    for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
-      for (int j = j0; j < j1; ++j) {
-        CT d2;
-        if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
-        const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
-        const CT inv = CT(1) / hab;
-        const CT q = sqrt(d2) * inv;
-        nr += (CT)S.m[j] * (w_shape<CT>(q) * h_dim_inv<CT>(inv, dim));
-      }
-    });
+    scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
+                      [&](int j, CT d2, CT, CT, CT) {
+                        const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
+                        const CT inv = CT(1) / hab;
+                        const CT q = sqrt(d2) * inv;
+                        nr += (CT)S.m[j] * (w_shape<CT>(q) * h_dim_inv<CT>(inv, dim));
+                      });
     rho += nr;
   }
   rho *= (CT)a.fac1;
@@ -90,20 +110,17 @@ __global__ __launch_bounds__(kBlock) void density_rate_kernel(Grid g, PassArgs<T
   for (int s = 0; s < a.nsrc; ++s) {
     const View<T> S = a.src[s];
     CT nr = CT(0);
-    for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
-      for (int j = j0; j < j1; ++j) {
-        CT d2;
-        if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
-        const CT hab = CT(0.5) * (ha + (CT)S.h[j]);
-        const CT inv = CT(1) / hab;
-        const CT rr = sqrt(d2);
-        const CT q = rr * inv;
-        const CT gf = grad_shape<CT>(q, rr, hab, inv) * h_dim_inv<CT>(inv, dim);
-        const CT rx = (CT)xd - (CT)S.x[j], ry = (CT)yd - (CT)S.y[j], rz = (CT)zd - (CT)S.z[j];
-        const CT vx = (CT)ud - (CT)S.u[j], vy = (CT)vd - (CT)S.v[j], vz = (CT)wd - (CT)S.w[j];
-        nr += (vx * rx + vy * ry + vz * rz) * gf * (CT)S.m[j];
-      }
-    });
+    scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
+                      [&](int j, CT d2, CT rx, CT ry, CT rz) {
+                        const CT hab = CT(0.5) * (ha + (CT)S.h[j]);
+                        const CT inv = CT(1) / hab;
+                        const CT rr = sqrt(d2);
+                        const CT q = rr * inv;
+                        const CT gf = grad_shape<CT>(q, rr, hab, inv) * h_dim_inv<CT>(inv, dim);
+                        const CT vx = (CT)ud - (CT)S.u[j], vy = (CT)vd - (CT)S.v[j],
+                                 vz = (CT)wd - (CT)S.w[j];
+                        nr += (vx * rx + vy * ry + vz * rz) * gf * (CT)S.m[j];
+                      });
     rate += nr;
   }
   a.rate[i] = (T)(rate * (CT)a.fac1);
@@ -161,17 +178,13 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
       if (!do_m && !do_x) continue;
       CT am[3] = {CT(0), CT(0), CT(0)};
       CT ax[3] = {CT(0), CT(0), CT(0)};
-      for_each_row(g, S.cstart, cx, cy, cz, ring, [&](int j0, int j1) {
-        for (int j = j0; j < j1; ++j) {
-          CT d2;
-          if (!pair_test<T, CT>(a, S, j, xd, yd, zd, r2f, r2d, d2)) continue;
+      scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
+                        [&](int j, CT d2, CT rx, CT ry, CT rz) {
           const CT hab = CT(0.5) * (ha + (CT)S.h[j]);
           const CT inv = CT(1) / hab;
           const CT rr = sqrt(d2);
           const CT q = rr * inv;
           const CT fac = h_dim_inv<CT>(inv, dim);
-          const CT rx = (CT)xd - (CT)S.x[j], ry = (CT)yd - (CT)S.y[j],
-                   rz = (CT)zd - (CT)S.z[j];
           const CT mb = (CT)S.m[j];
           const CT rhob = (CT)S.rho[j];
           if (do_m) {
@@ -198,7 +211,6 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
             ax[1] += temp * ((CT)S.v[j] - (CT)vd);
             ax[2] += temp * ((CT)S.w[j] - (CT)wd);
           }
-        }
       });
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
@@ -350,7 +362,8 @@ static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params*
   const bool do_tait = p != nullptr && cs != nullptr;
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
-  if (packed_inputs(a, false) && !a.symmetric) {
+  SPH_TRY(check_period(g, rmax, "sph_density"));
+  if (packed_inputs(a, false) && !a.symmetric && !G.periodic) {
     if (nl != nullptr) {
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
@@ -431,7 +444,8 @@ static int32_t momentum_impl(hipStream_t st, int mode, const sph_grid* g, const 
   }
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
-  if (packed_inputs(a, true) && !a.symmetric) {
+  SPH_TRY(check_period(g, rmax, who));
+  if (packed_inputs(a, true) && !a.symmetric && !G.periodic) {
     if (nl != nullptr) {
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
@@ -460,6 +474,7 @@ static int32_t density_rate_impl(hipStream_t st, const sph_grid* g, const sph_pa
   a.rate = (T*)rate;
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
+  SPH_TRY(check_period(g, rmax, "sph_density_rate"));
   const int nb = nblocks_for(a.dst.n);
   if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
     hipLaunchKernelGGL((density_rate_kernel<T, double>), dim3(nb), dim3(kBlock), 0, st, G, a);
