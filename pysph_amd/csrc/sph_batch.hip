@@ -79,6 +79,7 @@ __device__ __forceinline__ void for_each_tile(int mode, int ntiles, const int* _
   }
 }
 
+template <int DIM>
 __global__ __launch_bounds__(kBlock) void batch_density_kernel(
     const SceneArgs* __restrict__ args, int phase, int set, const Tile* __restrict__ tiles,
     int ntiles, const int* __restrict__ seg, int mode, int grab, int* counters) {
@@ -89,12 +90,12 @@ __global__ __launch_bounds__(kBlock) void batch_density_kernel(
     const Tile T = tiles[t];
     const int s = __builtin_amdgcn_readfirstlane(T.scene);
     const SceneArgs& A = args[s];
-    density_scan_body<float, true, false>(A.g, A.dens[phase][set],
-                                          __builtin_amdgcn_readfirstlane(T.start) + lane, q);
+    density_scan_body<float, true, false, DIM>(A.g, A.dens[phase][set],
+                                               __builtin_amdgcn_readfirstlane(T.start) + lane, q);
   });
 }
 
-template <int MODE>
+template <int MODE, int DIM>
 __global__ __launch_bounds__(kBlock) void batch_stage_kernel(
     const SceneArgs* __restrict__ args, const Tile* __restrict__ tiles, int ntiles,
     const int* __restrict__ seg, int mode, int grab, int* counters) {
@@ -105,8 +106,8 @@ __global__ __launch_bounds__(kBlock) void batch_stage_kernel(
     const Tile T = tiles[t];
     const int s = __builtin_amdgcn_readfirstlane(T.scene);
     const SceneArgs& A = args[s];
-    momentum_scan_body<float, MODE>(A.g, A.stage[MODE - 1],
-                                    __builtin_amdgcn_readfirstlane(T.start) + lane, q);
+    momentum_scan_body<float, MODE, DIM>(A.g, A.stage[MODE - 1],
+                                         __builtin_amdgcn_readfirstlane(T.start) + lane, q);
   });
 }
 
@@ -529,9 +530,14 @@ int32_t sph_batch_density(void* handle, void* stream, int32_t phase, int32_t set
   hipStream_t st = as_stream(stream);
   int* ctr = b->d_counters + (phase * 2 + set) * kXcds;
   if (sched->mode == 2) SPH_TRY(fill_i32(st, ctr, kXcds, 0));
-  hipLaunchKernelGGL(batch_density_kernel, dim3(launch_blocks(b, sched, nt)), dim3(kBlock), 0,
-                     st, b->d_args, phase, set, b->d_tiles[set][sched->order], nt,
-                     b->d_seg[set][sched->order], sched->mode, sched->grab, ctr);
+  if (b->dim == 3)
+    hipLaunchKernelGGL(batch_density_kernel<3>, dim3(launch_blocks(b, sched, nt)), dim3(kBlock),
+                       0, st, b->d_args, phase, set, b->d_tiles[set][sched->order], nt,
+                       b->d_seg[set][sched->order], sched->mode, sched->grab, ctr);
+  else
+    hipLaunchKernelGGL(batch_density_kernel<2>, dim3(launch_blocks(b, sched, nt)), dim3(kBlock),
+                       0, st, b->d_args, phase, set, b->d_tiles[set][sched->order], nt,
+                       b->d_seg[set][sched->order], sched->mode, sched->grab, ctr);
   return launch_status("sph_batch_density");
 }
 
@@ -550,12 +556,17 @@ int32_t sph_batch_stage(void* handle, void* stream, int32_t stage, const sph_bat
   const int nbk = launch_blocks(b, sched, nt);
   const Tile* tl = b->d_tiles[0][sched->order];
   const int* sg = b->d_seg[0][sched->order];
-  if (stage == 1)
-    hipLaunchKernelGGL(batch_stage_kernel<1>, dim3(nbk), dim3(kBlock), 0, st, b->d_args, tl, nt,
-                       sg, sched->mode, sched->grab, ctr);
-  else
-    hipLaunchKernelGGL(batch_stage_kernel<2>, dim3(nbk), dim3(kBlock), 0, st, b->d_args, tl, nt,
-                       sg, sched->mode, sched->grab, ctr);
+#define SPH_STAGE(M, D)                                                                   \
+  hipLaunchKernelGGL((batch_stage_kernel<M, D>), dim3(nbk), dim3(kBlock), 0, st, b->d_args, tl, \
+                     nt, sg, sched->mode, sched->grab, ctr)
+  if (stage == 1) {
+    if (b->dim == 3) SPH_STAGE(1, 3);
+    else SPH_STAGE(1, 2);
+  } else {
+    if (b->dim == 3) SPH_STAGE(2, 3);
+    else SPH_STAGE(2, 2);
+  }
+#undef SPH_STAGE
   return launch_status("sph_batch_stage");
 }
 

@@ -44,6 +44,13 @@ constexpr int kQueue = SPH_QUEUE;
 constexpr int kUnroll = SPH_UNROLL;
 __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
+// float4 gather at a non-negative index as a 32-bit byte offset from the
+// array base, so the load takes the scalar-base + 32-bit-offset form
+// (arrays stay below 2^28 particles).
+__device__ __forceinline__ float4 ld4(const float4* __restrict__ base, int j) {
+  return *(const float4*)((const char*)base + (unsigned)j * 16u);
+}
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
@@ -56,13 +63,6 @@ __device__ __forceinline__ bool accept_exact(const float4& s, const float4& d, f
   const float dx = s.x - d.x, dy = s.y - d.y, dz = s.z - d.z;
   const float d2f = dx * dx + dy * dy + dz * dz;
   return Within32::test(s.x, s.y, s.z, d.x, d.y, d.z, d2f, r2f, r2d);
-}
-
-// Superset test of the list build: inside radius*(1+skin), with a margin
-// far above FP32 rounding.
-__device__ __forceinline__ bool accept_super(const float4& s, const float4& d, float rl2f) {
-  const float dx = s.x - d.x, dy = s.y - d.y, dz = s.z - d.z;
-  return dx * dx + dy * dy + dz * dz < rl2f;
 }
 
 template <typename CT>
@@ -114,7 +114,7 @@ __device__ __forceinline__ CT g_sel(CT q, CT r, CT h, CT inv) {
 // stencil rows with geometric culling
 // ---------------------------------------------------------------------------
 struct Cull {
-  float xd, yd, zd, rc2;
+  float xd, yd, zd, rc2, r;
 };
 
 // Distance from v to the extent of cell c along one axis; border cells
@@ -135,7 +135,26 @@ __device__ __forceinline__ Cull make_cull(const float4& pd, double rc, const Gri
   const float scale = fabsf(pd.x) + fabsf(pd.y) + fabsf(pd.z) + g.cs_f;
   const float r = (float)rc * 1.00001f + 8e-7f * scale;
   c.rc2 = r * r;
+  c.r = r;
   return c;
+}
+
+// Stencil offsets [a, b] (within +-ring) along one axis whose cells can
+// hold a point within the cull radius of v; border cells reach to infinity.
+// The margin of the cull radius covers the FP32 rounding of the bounds, as
+// in row_range_cull.
+__device__ __forceinline__ void axis_span(float v, float r, int c, int n, int lo, float inv_cs,
+                                          int ring, int& a, int& b) {
+  const float last = (float)(n - 1);
+  const float fa = fminf(fmaxf(floorf((v - r) * inv_cs) - (float)lo, 0.f), last);
+  const float fb = fminf(fmaxf(floorf((v + r) * inv_cs) - (float)lo, 0.f), last);
+  if (!(fa <= fb)) {  // NaN positions keep the ring
+    a = -ring;
+    b = ring;
+    return;
+  }
+  a = max((int)fa - c, -ring);
+  b = min((int)fb - c, ring);
 }
 
 // Particle range of the cells of grid row (y, z) within the stencil ring of
@@ -149,7 +168,8 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   const float gz = axis_gap(cu.zd, z, g.dims[2], g.lo[2], g.cs_f);
   const float w2 = cu.rc2 - gy * gy - gz * gz;
   if (!(w2 >= 0.f)) return;
-  const float w = sqrtf(w2);
+  // 1-ulp hardware square root: the cull radius carries a 1e-5 margin
+  const float w = __builtin_amdgcn_sqrtf(w2);
   // cells of [x - w, x + w]; the border cells reach to infinity (particles
   // outside the grid are clamped into them), so clamp the range into the grid
   const float last = (float)(g.dims[0] - 1);
@@ -162,6 +182,52 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   const int base = (z * g.dims[1] + y) * g.dims[0];
   j0 = cstart[base + x0];
   j1 = cstart[base + x1 + 1];
+}
+
+// Screen the candidates j0 .. j0+kUnroll-1 (those below j1) of one stencil
+// row against destination pd and push the accepted ones into the lane's
+// LDS queue q (entries 64 apart), branch-free: every lane loads kUnroll
+// candidates (index 0 stands in past the row end; the source is not empty
+// while any lane has candidates), writes every candidate into the next
+// queue slot and advances its count only on acceptance.  The caller keeps
+// cnt <= kQueue - kUnroll on entry.  The exact test (Within32) runs its FP64
+// branch only when a candidate of some lane falls in the +-2e-6 band; with
+// `super` the test is the list-build superset d2 < rl2f instead.
+__device__ __forceinline__ void screen_block(const View<float>& S, int j0, int j1,
+                                             const float4& pd, float r2lo, float r2hi,
+                                             double r2d, bool super, float rl2f, int* q,
+                                             int& cnt) {
+  float4 c[kUnroll];
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) c[u] = ld4(S.pos4, j0 + u < j1 ? j0 + u : 0);
+  bool ok[kUnroll], bnd[kUnroll];
+  bool band = false;
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
+    const float d2f = dx * dx + dy * dy + dz * dz;
+    const bool in = j0 + u < j1;
+    if (super) {
+      ok[u] = in && d2f < rl2f;
+      bnd[u] = false;
+    } else {
+      ok[u] = in && d2f < r2lo;
+      bnd[u] = in && !(d2f < r2lo) && !(d2f > r2hi);
+    }
+    band = band || bnd[u];
+  }
+  if (__any(band)) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (bnd[u])
+        ok[u] = exact_within((double)c[u].x, (double)c[u].y, (double)c[u].z, (double)pd.x,
+                             (double)pd.y, (double)pd.z, r2d);
+  }
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    q[cnt * 64] = j0 + u;
+    cnt += ok[u] ? 1 : 0;
+  }
 }
 
 // Whether a list built with the given skin still holds every neighbour of a
@@ -213,7 +279,9 @@ __device__ __forceinline__ void density_epilogue(const PassArgs<float>& a, int i
 // Cell scan of destination i (all lanes of the wave call it together; q
 // is the lane's LDS queue, kQueue ints 64 apart); BUILD also records the
 // list (superset radius).
-template <typename CT, bool TAIT, bool BUILD>
+// DIM > 0 fixes the dimension at compile time (the kernels of dimensions 2
+// and 3); DIM = 0 reads it from the arguments.
+template <typename CT, bool TAIT, bool BUILD, int DIM = 0>
 __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<float>& a,
                                                   int i, int* q) {
   const bool valid = i < a.dst.n;
@@ -226,13 +294,17 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
   const bool exact_list = !BUILD || a.nl_skin == 0.f;
   const double rl = BUILD ? r * (1.0 + (double)a.nl_skin) : r;
   const float rl2f = (float)(rl * rl) * 1.00001f;
+  const float r2lo = r2f * 0.999998f, r2hi = r2f * 1.000002f;
   const bool write_list = BUILD && local;
   int cx = 0, cy = 0, cz = 0;
   if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
   const int ring = ring_of(g, a.disp);
   const double dd = a.disp ? (double)*a.disp : 0.0;
   const Cull cu = make_cull(pd, rl + dd, g);
-  const int dim = a.dim;
+  int dylo, dyhi, dzlo, dzhi;
+  axis_span(cu.yd, cu.r, cy, g.dims[1], g.lo[1], g.inv_cs_f, ring, dylo, dyhi);
+  axis_span(cu.zd, cu.r, cz, g.dims[2], g.lo[2], g.inv_cs_f, ring, dzlo, dzhi);
+  const int dim = DIM ? DIM : a.dim;
   const CT hd = (CT)pd.w;
   const int64_t ndst = a.dst.n;
   CT rho = CT(0);
@@ -242,13 +314,12 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
     CT nr = CT(0);
     int cnt = 0, cnt_s = 0;
     auto drain = [&]() {
-      const int m = wave_max(cnt);
-      for (int k = 0; k < m; ++k) {
+      for (int k = 0; __any(k < cnt); ++k) {
         if (k < cnt) {
           const int j = q[k * 64];
-          const float4 ps = S.pos4[j];
+          const float4 ps = ld4(S.pos4, j);
           if (exact_list || accept_exact(ps, pd, r2f, r2d))
-            nr += w_term<CT>(pd, ps, hd, (CT)S.vel4[j].w, dim);
+            nr += w_term<CT>(pd, ps, hd, (CT)ld4(S.vel4, j).w, dim);
           if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
         }
       }
@@ -257,23 +328,16 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
       cnt = 0;
     };
     for (int dz = -ring; dz <= ring; ++dz) {
+      const bool zok = local && dz >= dzlo && dz <= dzhi;
+      if (!__any(zok)) continue;
       for (int dy = -ring; dy <= ring; ++dy) {
+        const bool yok = zok && dy >= dylo && dy <= dyhi;
+        if (!__any(yok)) continue;
         int j0 = 0, j1 = 0;
-        if (local) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
+        if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
           if (__any(cnt > kQueue - kUnroll)) drain();
-          float4 c[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u) c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : kFar4;
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
-            const bool ok =
-                exact_list ? accept_exact(c[u], pd, r2f, r2d) : accept_super(c[u], pd, rl2f);
-            if (j0 + u < j1 && ok) {
-              q[cnt * 64] = j0 + u;
-              ++cnt;
-            }
-          }
+          screen_block(S, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q, cnt);
           j0 += kUnroll;
         }
       }
@@ -309,11 +373,14 @@ __device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, 
   const CT mb = (CT)vs.w;
   const CT rhob = (CT)as.x;
   const CT irhoab = frcp<CT>(CT(0.5) * (D.rhoa + rhob));
+  // the velocity difference serves both sums (v_b - v_a = -(v_a - v_b)
+  // exactly), and every source field is read before the branches, so the
+  // three gathers of a pair go out together
+  const CT vx = (CT)D.vd.x - (CT)vs.x, vy = (CT)D.vd.y - (CT)vs.y, vz = (CT)D.vd.z - (CT)vs.z;
+  const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
+  const CT cab = CT(0.5) * (D.ca + (CT)as.z);
   if (do_m) {
-    const CT vx = (CT)D.vd.x - (CT)vs.x, vy = (CT)D.vd.y - (CT)vs.y, vz = (CT)D.vd.z - (CT)vs.z;
     const CT dot = vx * rx + vy * ry + vz * rz;
-    const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
-    const CT cab = CT(0.5) * (D.ca + (CT)as.z);
     const CT mu = hab * dot * frcp<CT>(d2 + D.eta2 * hab * hab);
     const CT piab = (-D.alpha * cab * mu + D.beta * mu * mu) * irhoab;
     const CT tmp = D.Aa + Ab + (dot < CT(0) ? piab : CT(0));
@@ -324,20 +391,20 @@ __device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, 
   }
   if (do_x) {
     const CT temp = mb * (w_sel<CT>(qq) * fac) * irhoab;
-    ax[0] += temp * ((CT)vs.x - (CT)D.vd.x);
-    ax[1] += temp * ((CT)vs.y - (CT)D.vd.y);
-    ax[2] += temp * ((CT)vs.z - (CT)D.vd.z);
+    ax[0] += temp * -vx;
+    ax[1] += temp * -vy;
+    ax[2] += temp * -vz;
   }
 }
 
-template <typename CT>
+template <typename CT, int DIM = 0>
 __device__ __forceinline__ MomDst<CT> mom_dst(const PassArgs<float>& a, int i, bool valid) {
   const float4 zero4 = make_float4(0.f, 0.f, 0.f, 0.f);
   MomDst<CT> D;
   D.pd = valid ? a.dst.pos4[i] : zero4;
   D.vd = valid ? a.dst.vel4[i] : zero4;
   const float4 ad = valid ? a.dst.aux4[i] : zero4;
-  D.dim = a.dim;
+  D.dim = DIM ? DIM : a.dim;
   D.ha = (CT)D.pd.w;
   D.rhoa = (CT)ad.x;
   D.ca = (CT)ad.z;
@@ -407,19 +474,23 @@ __device__ __forceinline__ void mom_epilogue(const PassArgs<float>& a, int i, bo
 
 // Cell scan with LDS queues (no list) for destination i; q as in
 // density_scan_body.
-template <typename CT, int MODE>
+template <typename CT, int MODE, int DIM = 0>
 __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs<float>& a,
                                                    int i, int* q) {
   const bool valid = i < a.dst.n;
   const bool local = valid && (a.dst.tag == nullptr || a.dst.tag[i] == 0);
-  const MomDst<CT> D = mom_dst<CT>(a, i, valid);
+  const MomDst<CT> D = mom_dst<CT, DIM>(a, i, valid);
   const double r = (double)D.pd.w * a.rs;
   const double r2d = r * r;
   const float r2f = (float)r2d;
+  const float r2lo = r2f * 0.999998f, r2hi = r2f * 1.000002f;
   int cx = 0, cy = 0, cz = 0;
   if (local) cell_xyz(a.dst.cell[i], g, cx, cy, cz);
   const int ring = ring_of(g, a.disp);
   const Cull cu = make_cull(D.pd, r + (a.disp ? (double)*a.disp : 0.0), g);
+  int dylo, dyhi, dzlo, dzhi;
+  axis_span(cu.yd, cu.r, cy, g.dims[1], g.lo[1], g.inv_cs_f, ring, dylo, dyhi);
+  axis_span(cu.zd, cu.r, cz, g.dims[2], g.lo[2], g.inv_cs_f, ring, dzlo, dzhi);
   CT acc[3] = {CT(0), CT(0), CT(0)};
   CT dv[3] = {CT(0), CT(0), CT(0)};
   for (int s = 0; s < a.nsrc; ++s) {
@@ -431,31 +502,25 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
     CT ax[3] = {CT(0), CT(0), CT(0)};
     int cnt = 0;
     auto drain = [&]() {
-      const int m = wave_max(cnt);
-      for (int k = 0; k < m; ++k) {
+      for (int k = 0; __any(k < cnt); ++k) {
         if (k < cnt) {
           const int j = q[k * 64];
-          mom_pair<CT>(D, S.pos4[j], S.vel4[j], S.aux4[j], do_m, do_x, am, ax);
+          mom_pair<CT>(D, ld4(S.pos4, j), ld4(S.vel4, j), ld4(S.aux4, j), do_m, do_x, am, ax);
         }
       }
       cnt = 0;
     };
     for (int dz = -ring; dz <= ring; ++dz) {
+      const bool zok = local && dz >= dzlo && dz <= dzhi;
+      if (!__any(zok)) continue;
       for (int dy = -ring; dy <= ring; ++dy) {
+        const bool yok = zok && dy >= dylo && dy <= dyhi;
+        if (!__any(yok)) continue;
         int j0 = 0, j1 = 0;
-        if (local) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
+        if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
           if (__any(cnt > kQueue - kUnroll)) drain();
-          float4 c[kUnroll];
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u) c[u] = (j0 + u < j1) ? S.pos4[j0 + u] : kFar4;
-#pragma unroll
-          for (int u = 0; u < kUnroll; ++u) {
-            if (j0 + u < j1 && accept_exact(c[u], D.pd, r2f, r2d)) {
-              q[cnt * 64] = j0 + u;
-              ++cnt;
-            }
-          }
+          screen_block(S, j0, j1, D.pd, r2lo, r2hi, r2d, false, 0.f, q, cnt);
           j0 += kUnroll;
         }
       }

@@ -5,12 +5,13 @@
 namespace sph {
 
 
-template <typename CT, bool TAIT, bool BUILD>
+template <typename CT, bool TAIT, bool BUILD, int DIM>
 __global__ __launch_bounds__(kBlock) void density_scan(Grid g, PassArgs<float> a) {
   __shared__ int qbuf[kBlock / 64][kQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);
-  density_scan_body<CT, TAIT, BUILD>(g, a, bid * kBlock + threadIdx.x, &qbuf[wid][0][lane]);
+  density_scan_body<CT, TAIT, BUILD, DIM>(g, a, bid * kBlock + threadIdx.x,
+                                          &qbuf[wid][0][lane]);
 }
 
 // Density from a list (exact test on every entry); lanes whose list is
@@ -74,12 +75,12 @@ __global__ __launch_bounds__(kBlock) void density_list(Grid g, PassArgs<float> a
   density_epilogue<TAIT>(a, i, local, (double)(rho * (CT)a.fac1));
 }
 
-template <typename CT, int MODE>
+template <typename CT, int MODE, int DIM>
 __global__ __launch_bounds__(kBlock) void momentum_scan(Grid g, PassArgs<float> a) {
   __shared__ int qbuf[kBlock / 64][kQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);
-  momentum_scan_body<CT, MODE>(g, a, bid * kBlock + threadIdx.x, &qbuf[wid][0][lane]);
+  momentum_scan_body<CT, MODE, DIM>(g, a, bid * kBlock + threadIdx.x, &qbuf[wid][0][lane]);
 }
 
 // List pass: exact test on every entry, two entries per iteration; lanes
@@ -172,23 +173,44 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __r
   if (vel4 != nullptr) vel4[i] = make_float4(a.u[i], a.v[i], a.w[i], a.m[i]);
 }
 
+// Compile-time dimension of the FP32-pair scan kernels (0: from the
+// arguments).
+template <typename CT>
+static int scan_dim(int dim) {
+  return sizeof(CT) == 4 && (dim == 2 || dim == 3) ? dim : 0;
+}
+
+template <typename CT, int DIM>
+static void density_scan_launch(hipStream_t st, int nb, const Grid& G, const PassArgs<float>& a,
+                                bool tait, bool build) {
+#define SPH_DENS(K) hipLaunchKernelGGL(K, dim3(nb), dim3(kBlock), 0, st, G, a)
+  if (build) {
+    if (tait) SPH_DENS((density_scan<CT, true, true, DIM>));
+    else SPH_DENS((density_scan<CT, false, true, DIM>));
+  } else {
+    if (tait) SPH_DENS((density_scan<CT, true, false, DIM>));
+    else SPH_DENS((density_scan<CT, false, false, DIM>));
+  }
+#undef SPH_DENS
+}
+
 template <typename CT>
 static void density_launch(hipStream_t st, const Grid& G, const PassArgs<float>& a, bool tait) {
   const int nb = nblocks_for(a.dst.n);
   const bool list = a.nl_idx != nullptr;
   const bool build = list && a.nl_mode == 0;
-#define SPH_DENS(K) hipLaunchKernelGGL(K, dim3(nb), dim3(kBlock), 0, st, G, a)
-  if (build) {
-    if (tait) SPH_DENS((density_scan<CT, true, true>));
-    else SPH_DENS((density_scan<CT, false, true>));
-  } else if (list) {
-    if (tait) SPH_DENS((density_list<CT, true>));
-    else SPH_DENS((density_list<CT, false>));
-  } else {
-    if (tait) SPH_DENS((density_scan<CT, true, false>));
-    else SPH_DENS((density_scan<CT, false, false>));
+  if (list && !build) {
+    if (tait)
+      hipLaunchKernelGGL((density_list<CT, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
+    else
+      hipLaunchKernelGGL((density_list<CT, false>), dim3(nb), dim3(kBlock), 0, st, G, a);
+    return;
   }
-#undef SPH_DENS
+  switch (scan_dim<CT>(a.dim)) {
+    case 2: density_scan_launch<CT, 2>(st, nb, G, a, tait, build); break;
+    case 3: density_scan_launch<CT, 3>(st, nb, G, a, tait, build); break;
+    default: density_scan_launch<CT, 0>(st, nb, G, a, tait, build); break;
+  }
 }
 
 int32_t fast_density(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,
@@ -207,24 +229,35 @@ int32_t fast_density(hipStream_t st, const sph_grid& g, double rmax, PassArgs<fl
   return launch_status("sph_density");
 }
 
+template <typename CT, int MODE>
+static void momentum_launch_mode(hipStream_t st, int nb, const Grid& G,
+                                 const PassArgs<float>& a) {
+  if (a.nl_idx != nullptr) {
+    hipLaunchKernelGGL((momentum_list<CT, MODE>), dim3(nb), dim3(kBlock), 0, st, G, a);
+    return;
+  }
+  switch (scan_dim<CT>(a.dim)) {
+    case 2:
+      hipLaunchKernelGGL((momentum_scan<CT, MODE, 2>), dim3(nb), dim3(kBlock), 0, st, G, a);
+      break;
+    case 3:
+      hipLaunchKernelGGL((momentum_scan<CT, MODE, 3>), dim3(nb), dim3(kBlock), 0, st, G, a);
+      break;
+    default:
+      hipLaunchKernelGGL((momentum_scan<CT, MODE, 0>), dim3(nb), dim3(kBlock), 0, st, G, a);
+      break;
+  }
+}
+
 template <typename CT>
 static void momentum_launch(hipStream_t st, const Grid& G, const PassArgs<float>& a, int mode) {
   const int nb = nblocks_for(a.dst.n);
-  const bool list = a.nl_idx != nullptr;
-#define SPH_MOM(M)                                                                         \
-  do {                                                                                     \
-    if (list)                                                                              \
-      hipLaunchKernelGGL((momentum_list<CT, M>), dim3(nb), dim3(kBlock), 0, st, G, a);     \
-    else                                                                                   \
-      hipLaunchKernelGGL((momentum_scan<CT, M>), dim3(nb), dim3(kBlock), 0, st, G, a);     \
-  } while (0)
   switch (mode) {
-    case 0: SPH_MOM(0); break;
-    case 1: SPH_MOM(1); break;
-    case 2: SPH_MOM(2); break;
-    default: SPH_MOM(3); break;
+    case 0: momentum_launch_mode<CT, 0>(st, nb, G, a); break;
+    case 1: momentum_launch_mode<CT, 1>(st, nb, G, a); break;
+    case 2: momentum_launch_mode<CT, 2>(st, nb, G, a); break;
+    default: momentum_launch_mode<CT, 3>(st, nb, G, a); break;
   }
-#undef SPH_MOM
 }
 
 int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,

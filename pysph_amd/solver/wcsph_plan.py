@@ -25,9 +25,12 @@ stays within the skin (checked on the device), and stage 2 keeps the
 binning.
     momentum + XSPH + final update -> X (in place)  sph_wcsph_stage(2)
 
-Stage 2 re-bins the moving array by the predicted positions X1 and carries
-X, X1 and k1 through the same permutation, as the reference rebuilds its
-neighbour caches after the predictor (integrator.py:621-645).  The update
+With stage2_rebin, stage 2 re-bins the moving array by the predicted
+positions X1 and carries X, X1 and k1 through the same permutation, as the
+reference rebuilds its neighbour caches after the predictor
+(integrator.py:621-645); by default the packed path keeps the stage-1
+binning and widens the stage-2 stencil by the predictor's displacement
+(same neighbour sets, one binning less per step).  The update
 order inside the stage kernels is the reference's per-calc order
 (mom, gravity, step, xsph), so results match the generic path.
 """
@@ -91,6 +94,13 @@ class WCSPHPlan:
         # skin * radius / 2; otherwise every stage builds exact lists
         self.list_reuse = False
         self.list_skin = 0.0
+        # stage2_rebin: re-bin the moving array by X1 before stage 2 (the
+        # reference rebuilds its caches after the predictor).  Off, stage 2
+        # keeps the stage-1 binning and the packed passes widen their
+        # stencil by the predictor's largest displacement (measured on the
+        # device by the stage-1 kernel), which skips one binning and
+        # permutation per step; the neighbour sets are the same.
+        self.stage2_rebin = False
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
 
@@ -337,11 +347,17 @@ class WCSPHPlan:
         b = self._buffers()
         self._pack_all()
         ov = self._override()
-        if self.use_list and self.list_reuse:
-            # stage 1 builds the neighbour lists (superset radius) that the
-            # stage-1 momentum pass and every stage-2 pass read; stage 2 keeps
-            # the binning and checks the predictor's displacement on the
-            # device against the list skin
+        # lists built anew in stage 2 need the X1 binning (their validity
+        # check reads the same displacement as the widened stencil)
+        keep_binning = (self.use_list and self.list_reuse) or (
+            self.use_packed and not self.stage2_rebin and not self.use_list)
+        if keep_binning:
+            # stage 2 keeps the binning: the stage-1 kernel records the
+            # predictor's largest displacement on the device, and the stage-2
+            # passes widen their stencil by it; with reused lists, stage 1
+            # builds the neighbour lists (superset radius) that the stage-1
+            # momentum pass and every stage-2 pass read, valid while that
+            # displacement stays within the list skin
             stream = require_gpu(b["disp"])
             _abi.call("sph_fill_i32", stream, b["disp"].data_ptr(), 1, 0)
             self._density(override=ov, list_mode=0)
