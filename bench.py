@@ -54,11 +54,14 @@ def parse():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--reuse", type=float, default=None, metavar="SKIN",
                     help="reuse stage-1 neighbour lists in stage 2 with this skin")
+    ap.add_argument("--cache-static", action="store_true",
+                    help="keep the boundary-from-boundary density sums between calls "
+                         "(same bits; those pairs are then evaluated once, not per stage)")
     return ap.parse_args()
 
 
 def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, nlist=False,
-               reuse=None):
+               reuse=None, cache_static=False):
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
     from pysph_amd.solver import generators as gen
@@ -82,6 +85,7 @@ def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, n
         plan.use_packed = plan.use_packed and packed
         if nlist or reuse is not None:
             plan.enable_lists(True, skin=reuse, reuse=reuse is not None)
+        plan.cache_static = bool(cache_static)
     return s, particles, f, b, info, col, tank
 
 
@@ -102,6 +106,7 @@ def pair_counts(particles, f, b):
             counts[(dst.name, src.name)] = int(c.sum().item())
     p_rho = sum(counts.values())
     p_m = counts[("fluid", "fluid")] + counts[("fluid", "boundary")]
+    pair_counts.boundary_boundary = counts[("boundary", "boundary")]
     # density pairs of the fluid destinations (= p_m: same source arrays)
     return p_rho, p_m, p_m
 
@@ -110,7 +115,8 @@ def config_key(args):
     """Kernel configuration the traffic figures belong to."""
     return "cpr%d-%s-%s%s" % (args.cells_per_radius, "packed" if not args.no_packed else "soa",
                               "list" if args.list else "nolist",
-                              "" if args.reuse is None else "-reuse%g" % args.reuse)
+                              "" if args.reuse is None else "-reuse%g" % args.reuse) + (
+                                  "-cache" if getattr(args, "cache_static", False) else "")
 
 
 def kernel_names(plan):
@@ -154,7 +160,7 @@ class LaunchTimer:
         self.cur = None
 
     def key(self, name, args):
-        if name == "sph_density":
+        if name in ("sph_density", "sph_density_cached"):
             return "density_fluid" if args[3][0].n == self.nf else "density_boundary"
         if name == "sph_wcsph_stage":
             return "stage%d" % args[1]
@@ -712,7 +718,8 @@ def main():
     s, particles, f, b, info, col, tank = build_case(args.scale,
                                                      cells_per_radius=args.cells_per_radius,
                                                      packed=not args.no_packed,
-                                                     nlist=args.list, reuse=args.reuse)
+                                                     nlist=args.list, reuse=args.reuse,
+                                                     cache_static=args.cache_static)
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
@@ -778,6 +785,11 @@ def main():
                    "fluid_lattice": list(col), "tank_cells": list(tank),
                    "n_fluid": nf, "n_boundary": nb, "dx": info["dx"], "h": info["h"],
                    "dt": info["dt"], "c0": info["co"], "pairs_per_step": pairs_per_step,
+                   "cache_static": args.cache_static,
+                   # with the static cache the boundary-from-boundary density
+                   # pairs are evaluated once, not in every stage
+                   "pairs_evaluated_per_step": pairs_per_step - (
+                       2 * pair_counts.boundary_boundary if args.cache_static else 0),
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": names[key], "pass": key,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

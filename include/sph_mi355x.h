@@ -250,6 +250,29 @@ int32_t sph_density_rate(void *stream, const sph_grid *g, const sph_params *prm,
                          const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                          double rmax, const float *disp_dev, void *rate_dev);
 
+/* sph_density with the partial sums of one source kept between calls.
+ * When a source does not move relative to the destination (boundary
+ * particles from boundary particles), its contribution to each
+ * destination's density is the same at every call: mode SPH_CACHE_FILL
+ * computes it as usual and also stores it in sums_dev (one double per
+ * destination); mode SPH_CACHE_USE adds the stored value instead of
+ * scanning that source.  The partial sum enters the total in the same
+ * position, so the result has the same bits.  The caller returns to
+ * SPH_CACHE_FILL whenever the destination or that source moved, changed
+ * mass or smoothing length, or was re-binned.  Not combinable with a
+ * neighbour list (nlist_out must be NULL). */
+#define SPH_CACHE_FILL 1
+#define SPH_CACHE_USE 2
+typedef struct sph_src_cache {
+    double *sums;
+    int32_t src;  /* index of the cached source in srcs */
+    int32_t mode; /* SPH_CACHE_FILL or SPH_CACHE_USE */
+} sph_src_cache;
+int32_t sph_density_cached(void *stream, const sph_grid *g, const sph_params *prm,
+                           const sph_view *dst, int32_t nsrc, const sph_view *srcs,
+                           double rmax, const float *disp_dev, void *rho_dev, void *p_dev,
+                           void *cs_dev, const sph_src_cache *cache);
+
 /* TaitEquation alone: p, cs from rho (eos_funcs.pyx:105-116). */
 int32_t sph_tait(void *stream, const sph_params *prm, const sph_view *a,
                  void *p_dev, void *cs_dev);

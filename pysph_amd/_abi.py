@@ -58,6 +58,14 @@ class SphNlist(ctypes.Structure):
                 ("mode", ctypes.c_int32), ("skin", ctypes.c_float), ("reserved", ctypes.c_int32)]
 
 
+class SphSrcCache(ctypes.Structure):
+    _fields_ = [("sums", _vp), ("src", ctypes.c_int32), ("mode", ctypes.c_int32)]
+
+
+SPH_CACHE_FILL = 1
+SPH_CACHE_USE = 2
+
+
 class SphUpdateOp(ctypes.Structure):
     _fields_ = [("out", _vp), ("a", _vp), ("b", _vp), ("c", _vp)]
 
@@ -148,6 +156,9 @@ _SIGS = {
     "sph_density_rate": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                           ctypes.c_int32, _P(SphView), ctypes.c_double, _vp,
                                           _vp]),
+    "sph_density_cached": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
+                                            ctypes.c_int32, _P(SphView), ctypes.c_double, _vp,
+                                            _vp, _vp, _vp, _P(SphSrcCache)]),
     "sph_tait": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp, _vp]),
     "sph_periodic_wrap": (ctypes.c_int32, [_vp, _P(SphGrid), ctypes.c_int32, ctypes.c_int64,
                                            _P(_vp), ctypes.c_int32, _P(_vp)]),

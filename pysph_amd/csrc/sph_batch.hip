@@ -96,7 +96,8 @@ __global__ __launch_bounds__(kBlock) void batch_density_kernel(
 }
 
 template <int MODE, int DIM>
-__global__ __launch_bounds__(kBlock) void batch_stage_kernel(
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMomWaves)))
+void batch_stage_kernel(
     const SceneArgs* __restrict__ args, const Tile* __restrict__ tiles, int ntiles,
     const int* __restrict__ seg, int mode, int grab, int* counters) {
   __shared__ int qbuf[kBlock / 64][kQueue][64];

@@ -41,6 +41,8 @@ namespace sph {
 #define SPH_UNROLL 4
 #endif
 constexpr int kQueue = SPH_QUEUE;
+// occupancy target of the momentum scan kernels (waves per SIMD)
+constexpr int kMomWaves = 6;
 constexpr int kUnroll = SPH_UNROLL;
 __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
@@ -312,6 +314,12 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
     CT nr = CT(0);
+    if (!BUILD && s == a.cache_src && a.cache_mode == 2) {
+      // stored partial sum of a static source (exact round trip via double)
+      if (local) nr = (CT)a.cache_sums[i];
+      rho += nr;
+      continue;
+    }
     int cnt = 0, cnt_s = 0;
     auto drain = [&]() {
       for (int k = 0; __any(k < cnt); ++k) {
@@ -343,6 +351,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
       }
     }
     drain();
+    if (!BUILD && s == a.cache_src && a.cache_mode == 1 && valid) a.cache_sums[i] = (double)nr;
     rho += nr;
     if (BUILD && valid) a.nl_count[(int64_t)s * ndst + i] = local ? cnt_s : 0;
   }

@@ -75,8 +75,12 @@ __global__ __launch_bounds__(kBlock) void density_list(Grid g, PassArgs<float> a
   density_epilogue<TAIT>(a, i, local, (double)(rho * (CT)a.fac1));
 }
 
+// Six waves per SIMD: the compiler keeps the momentum scan within 80 VGPRs
+// (it takes 88 unconstrained, five waves), without spilling; measured 7%
+// faster on the 3D dam break.
 template <typename CT, int MODE, int DIM>
-__global__ __launch_bounds__(kBlock) void momentum_scan(Grid g, PassArgs<float> a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(kMomWaves)))
+void momentum_scan(Grid g, PassArgs<float> a) {
   __shared__ int qbuf[kBlock / 64][kQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);

@@ -79,6 +79,10 @@ struct PassArgs {
   float nl_skin;
   int symmetric;  // variable-h symmetric pairs
   T* rate;        // SPHDensityRate output
+  // density: stored partial sums of source cache_src (mode 1 fill, 2 use)
+  double* cache_sums;
+  int cache_src;
+  int cache_mode;
 };
 
 // Scheme parameters of a pass from the ABI struct.

@@ -66,6 +66,10 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
   for (int s = 0; s < a.nsrc; ++s) {
     const View<T> S = a.src[s];
     CT nr = CT(0);
+    if (s == a.cache_src && a.cache_mode == 2) {
+      rho += (CT)a.cache_sums[i];  // stored partial sum of a static source
+      continue;
+    }
     scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
                       [&](int j, CT d2, CT, CT, CT) {
                         const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
@@ -73,6 +77,7 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
                         const CT q = sqrt(d2) * inv;
                         nr += (CT)S.m[j] * (w_shape<CT>(q) * h_dim_inv<CT>(inv, dim));
                       });
+    if (s == a.cache_src && a.cache_mode == 1) a.cache_sums[i] = (double)nr;
     rho += nr;
   }
   rho *= (CT)a.fac1;
@@ -351,10 +356,16 @@ template <typename T>
 static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params* prm,
                             const sph_view* dst, int32_t nsrc, const sph_view* srcs,
                             double rmax, const float* disp, void* rho, void* p, void* cs,
-                            const sph_nlist* nl) {
+                            const sph_nlist* nl, const sph_src_cache* cache) {
   PassArgs<T> a{};
   SPH_TRY(fill_views<T>(a, dst, nsrc, srcs, "sph_density"));
   set_params(a, prm);
+  a.cache_src = -1;
+  if (cache != nullptr) {
+    a.cache_sums = cache->sums;
+    a.cache_src = cache->src;
+    a.cache_mode = cache->mode;
+  }
   a.disp = disp;
   a.rho = (T*)rho;
   a.p = (T*)p;
@@ -504,10 +515,19 @@ int32_t sph_density_rate(void* stream, const sph_grid* g, const sph_params* prm,
                                    rate_dev);
 }
 
-int32_t sph_density(void* stream, const sph_grid* g, const sph_params* prm, const sph_view* dst,
-                    int32_t nsrc, const sph_view* srcs, double rmax, const float* disp_dev,
-                    void* rho_dev, void* p_dev, void* cs_dev, const sph_nlist* nlist_out) {
+static int32_t density_entry(void* stream, const sph_grid* g, const sph_params* prm,
+                             const sph_view* dst, int32_t nsrc, const sph_view* srcs,
+                             double rmax, const float* disp_dev, void* rho_dev, void* p_dev,
+                             void* cs_dev, const sph_nlist* nlist_out,
+                             const sph_src_cache* cache) {
   SPH_TRY(check_common(g, prm, dst, nsrc, srcs, false, "sph_density"));
+  if (cache != nullptr &&
+      (nlist_out != nullptr || cache->src < 0 || cache->src >= nsrc ||
+       (cache->mode != SPH_CACHE_FILL && cache->mode != SPH_CACHE_USE) ||
+       (dst->n > 0 && cache->sums == nullptr))) {
+    set_error("sph_density: bad source cache (a valid source, mode 1 or 2, sums, no list)");
+    return SPH_EINVAL;
+  }
   if (dst->n > 0 && rho_dev == nullptr) {
     set_error("sph_density: NULL rho output");
     return SPH_EINVAL;
@@ -522,9 +542,24 @@ int32_t sph_density(void* stream, const sph_grid* g, const sph_params* prm, cons
   }
   if (dst->dtype == SPH_F32)
     return density_impl<float>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,
-                               rho_dev, p_dev, cs_dev, nlist_out);
+                               rho_dev, p_dev, cs_dev, nlist_out, cache);
   return density_impl<double>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,
-                              rho_dev, p_dev, cs_dev, nlist_out);
+                              rho_dev, p_dev, cs_dev, nlist_out, cache);
+}
+
+int32_t sph_density(void* stream, const sph_grid* g, const sph_params* prm, const sph_view* dst,
+                    int32_t nsrc, const sph_view* srcs, double rmax, const float* disp_dev,
+                    void* rho_dev, void* p_dev, void* cs_dev, const sph_nlist* nlist_out) {
+  return density_entry(stream, g, prm, dst, nsrc, srcs, rmax, disp_dev, rho_dev, p_dev, cs_dev,
+                       nlist_out, nullptr);
+}
+
+int32_t sph_density_cached(void* stream, const sph_grid* g, const sph_params* prm,
+                           const sph_view* dst, int32_t nsrc, const sph_view* srcs, double rmax,
+                           const float* disp_dev, void* rho_dev, void* p_dev, void* cs_dev,
+                           const sph_src_cache* cache) {
+  return density_entry(stream, g, prm, dst, nsrc, srcs, rmax, disp_dev, rho_dev, p_dev, cs_dev,
+                       nullptr, cache);
 }
 
 int32_t sph_tait(void* stream, const sph_params* prm, const sph_view* a, void* p_dev,

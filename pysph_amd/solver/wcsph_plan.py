@@ -101,6 +101,12 @@ class WCSPHPlan:
         # device by the stage-1 kernel), which skips one binning and
         # permutation per step; the neighbour sets are the same.
         self.stage2_rebin = False
+        # cache_static (opt-in): the density of a non-moving array keeps the
+        # partial sums of its non-moving sources (boundary from boundary)
+        # between calls while neither is re-binned (same bits, fewer pairs
+        # evaluated per step)
+        self.cache_static = False
+        self._caches = {}
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
 
@@ -302,6 +308,32 @@ class WCSPHPlan:
         st.skin = self.list_skin
         return st
 
+    def _src_cache(self, c):
+        """SphSrcCache for a density calc whose destination and one source
+        do not move (boundary from boundary): the source's partial sums are
+        computed once and reused while neither array is re-binned."""
+        pa = c.dest
+        if not self.cache_static or pa is self.fluid:
+            return None
+        k = next((i for i, s in enumerate(c.sources) if s is not self.fluid), None)
+        if k is None:
+            return None
+        src = c.sources[k]
+        n = pa.get_number_of_particles()
+        key = (pa.__dict__.get("_bin_version", 0), src.__dict__.get("_bin_version", 0),
+               self.cm.grid_key(self.cm.grid), n)
+        ent = self._caches.get(id(c))
+        if ent is None or ent["sums"].numel() < max(n, 1):
+            ent = {"sums": torch.empty(max(n, 1), dtype=torch.float64, device=pa.device),
+                   "key": None}
+            self._caches[id(c)] = ent
+        st = _abi.SphSrcCache()
+        st.sums = ent["sums"].data_ptr()
+        st.src = k
+        st.mode = _abi.SPH_CACHE_USE if ent["key"] == key else _abi.SPH_CACHE_FILL
+        ent["key"] = key
+        return st
+
     def _density(self, disp=None, override=None, list_mode=0):
         prm = self.prm
         for c in self.rho_calcs:
@@ -309,12 +341,14 @@ class WCSPHPlan:
             rho = pa.get_carray("rho")
             # only the moving array has a momentum pass that reads a list
             nl = self._nlist(pa, list_mode) if pa is self.fluid else None
+            cache = self._src_cache(c) if nl is None else None
             if id(pa) in self.tait_arrays:
                 _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
-                             pa.get_carray("cs"), disp=disp, override=override, nl=nl)
+                             pa.get_carray("cs"), disp=disp, override=override, nl=nl,
+                             cache=cache)
             else:
                 _ops.density(self.cm, prm, pa, c.sources, rho, disp=disp, override=override,
-                             nl=nl)
+                             nl=nl, cache=cache)
 
     def _stage(self, stage, dt, b, override=None, disp_in=None, disp_out=None):
         pa = self.fluid

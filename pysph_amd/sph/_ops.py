@@ -101,15 +101,22 @@ def packed_override(arrays, override=None):
     return ov
 
 
-def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, nl=None):
+def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, nl=None,
+            cache=None):
     """SPHRho over srcs into rho (+ Tait into p, cs); `nl` (SphNlist)
-    receives the accepted neighbours for a following momentum pass."""
+    receives the accepted neighbours for a following momentum pass;
+    `cache` (SphSrcCache) keeps the partial sums of one static source."""
     check_same_dtype([dst] + list(srcs))
     stream = require_gpu(rho)
     if AUTO_PACK and dst.dtype == torch.float32 and nl is None:
         override = packed_override([dst] + [s for s in srcs if s is not dst], override)
     vd = views([dst], override)
     vs = views(srcs, override)
+    if cache is not None:
+        _abi.call("sph_density_cached", stream, cm.grid, prm, vd, len(srcs), vs,
+                  search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs),
+                  cache)
+        return
     _abi.call("sph_density", stream, cm.grid, prm, vd, len(srcs), vs,
               search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs), nl)
 

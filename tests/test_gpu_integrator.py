@@ -244,3 +244,27 @@ def test_fused_rk2_3d_step_vs_oracle():
     assert vel_err < 1e-5
     for c in "xyz":
         assert np.max(np.abs(f.get(c) - rf[c])) < 2e-7
+
+
+@pytest.mark.parametrize("precision,packed", [("single", True), ("double", False)])
+def test_static_source_cache_gives_same_bits(precision, packed):
+    """The boundary density keeps its boundary-from-boundary partial sums
+    between calls (sph_density_cached); every field stays bit-identical to
+    recomputing them, over steps that re-use and refill the cache."""
+    out = []
+    for cache in (False, True):
+        s, f, b, info, particles = _dam3d_solver(precision=precision, packed=packed,
+                                                 cells_per_radius=2, nlist=False)
+        plan = s.integrator.plan
+        plan.cache_static = cache
+        for k in range(6):
+            if k == 3:
+                # a re-binning of the boundary invalidates the stored sums
+                b.set_dirty(True)
+            s.integrator.integrate(info["dt"])
+        out.append({c: f.get(c) for c in ("x", "y", "z", "u", "v", "w", "rho", "p")})
+        out[-1]["brho"] = b.get("rho")
+        if cache:
+            assert plan._caches, "the boundary density calc keeps a cache"
+    for c in out[0]:
+        assert np.array_equal(out[0][c], out[1][c]), c
