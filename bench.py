@@ -623,7 +623,7 @@ def main_surrogate(args):
         model.forward(g)
     ms, calls = model.timing(0)
     per = {k: v / calls for k, v in ms.items()}
-    n, ne = g.n, g.nedges
+    n, ne = g.n, g.nedges_exact()
     gemm_flops = 2.0 * n * (16 * 128 + 2 * (128 * 256 + 256 * 128))
     # edge aggregation, per round: P, Q rows (2 x 512 B) and the BF16
     # aggregate (256 B) per node, the 16-float feature row per edge

@@ -189,6 +189,17 @@ int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
                           int64_t *nbrs_dev, int64_t cap, int64_t total,
                           int64_t *needed_host);
 
+/* sph_neighbor_fill without a host read-back of the total (no
+ * synchronisation): offsets_dev come from sph_scan_counts with a NULL
+ * total; entries at positions >= cap are not written and set
+ * *overflow_dev to 1 (the caller zeroes it and checks it later).  For
+ * callers that size the CSR from a known bound, such as a graph rebuilt
+ * every step. */
+int32_t sph_neighbor_fill_capped(void *stream, const sph_grid *g, const sph_view *dst,
+                                 const sph_view *src, double radius_scale,
+                                 const int64_t *src_ids_dev, const int64_t *offsets_dev,
+                                 int64_t *nbrs_dev, int64_t cap, int32_t *overflow_dev);
+
 /* ---- VarHNbrParticleLocator (nnps.pyx:654-749) as a CSR export: the
  *      symmetric set d2 < (h_d*rs)^2 OR d2 < (h_s*rs)^2, rows sorted
  *      ascending like the FixedDest export.  rmax = radius_scale * the

@@ -147,6 +147,9 @@ _SIGS = {
     "sph_density": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                      ctypes.c_int32, _P(SphView), ctypes.c_double, _vp, _vp,
                                      _vp, _vp, _P(SphNlist)]),
+    "sph_neighbor_fill_capped": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
+                                                  ctypes.c_double, _vp, _vp, _vp,
+                                                  ctypes.c_int64, _vp]),
     "sph_varh_neighbor_counts": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
                                                   ctypes.c_double, ctypes.c_double, _vp]),
     "sph_varh_neighbor_fill": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),

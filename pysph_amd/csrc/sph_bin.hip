@@ -308,6 +308,8 @@ struct NbrArgs {
   int64_t* out;
   int64_t* counts;
   double rmax_sym;  // > 0: symmetric variable-h sets, stencil for this radius
+  int64_t cap;      // fill: entries at positions >= cap are dropped...
+  int32_t* overflow;  // ... and flagged here (nullable: cap is not checked)
 };
 
 template <typename T>
@@ -365,7 +367,7 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
                              (double)xd, (double)yd, (double)zd, r2);
         }
         if (hit) {
-          if (FILL) a.out[k] = a.ids[j];
+          if (FILL && (a.overflow == nullptr || k < a.cap)) a.out[k] = a.ids[j];
           ++k;
         }
       }
@@ -374,6 +376,10 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   if (!FILL) {
     a.counts[i] = k;
   } else {
+    if (a.overflow != nullptr && k > a.cap) {
+      *a.overflow = 1;  // plain vector store of a flag
+      k = a.cap;
+    }
     // sort the row ascending (rows hold tens of entries)
     const int64_t b = a.off[i];
     for (int64_t p = b + 1; p < k; ++p) {
@@ -610,14 +616,17 @@ static int32_t nbr_fill(void* stream, const sph_grid* g, const sph_view* dst,
                         const sph_view* src, double radius_scale, double rmax_sym,
                         const int64_t* src_ids_dev, const int64_t* offsets_dev,
                         int64_t* nbrs_dev, int64_t cap, int64_t total, int64_t* needed_host,
-                        const char* who) {
+                        const char* who, int32_t* overflow = nullptr) {
   SPH_TRY(nbr_common(g, dst, src, radius_scale, who));
-  if (needed_host) *needed_host = total;
-  if (cap < total) {
-    set_error("%s: capacity %lld < %lld pairs", who, (long long)cap, (long long)total);
-    return SPH_ECAPACITY;
+  if (overflow == nullptr) {
+    if (needed_host) *needed_host = total;
+    if (cap < total) {
+      set_error("%s: capacity %lld < %lld pairs", who, (long long)cap, (long long)total);
+      return SPH_ECAPACITY;
+    }
+    if (total == 0) return SPH_OK;
   }
-  if (dst->n == 0 || total == 0) return SPH_OK;
+  if (dst->n == 0) return SPH_OK;
   if (!src_ids_dev || !offsets_dev || !nbrs_dev) {
     set_error("%s: NULL argument", who);
     return SPH_EINVAL;
@@ -627,14 +636,27 @@ static int32_t nbr_fill(void* stream, const sph_grid* g, const sph_view* dst,
   const int nb = nblocks_for(dst->n);
   if (dst->dtype == SPH_F32) {
     NbrArgs<float> a{to_view<float>(*dst), to_view<float>(*src), radius_scale, src_ids_dev,
-                     offsets_dev, nbrs_dev, nullptr, rmax_sym};
+                     offsets_dev, nbrs_dev, nullptr, rmax_sym, cap, overflow};
     hipLaunchKernelGGL((nbr_kernel<float, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
   } else {
     NbrArgs<double> a{to_view<double>(*dst), to_view<double>(*src), radius_scale, src_ids_dev,
-                      offsets_dev, nbrs_dev, nullptr, rmax_sym};
+                      offsets_dev, nbrs_dev, nullptr, rmax_sym, cap, overflow};
     hipLaunchKernelGGL((nbr_kernel<double, true>), dim3(nb), dim3(kBlock), 0, st, G, a);
   }
   return launch_status(who);
+}
+
+int32_t sph_neighbor_fill_capped(void* stream, const sph_grid* g, const sph_view* dst,
+                                 const sph_view* src, double radius_scale,
+                                 const int64_t* src_ids_dev, const int64_t* offsets_dev,
+                                 int64_t* nbrs_dev, int64_t cap, int32_t* overflow_dev) {
+  if (cap < 0 || overflow_dev == nullptr ||
+      (dst != nullptr && dst->n > 0 && (!src_ids_dev || !offsets_dev || !nbrs_dev))) {
+    set_error("sph_neighbor_fill_capped: bad arguments");
+    return SPH_EINVAL;
+  }
+  return nbr_fill(stream, g, dst, src, radius_scale, 0.0, src_ids_dev, offsets_dev, nbrs_dev,
+                  cap, -1, nullptr, "sph_neighbor_fill_capped", overflow_dev);
 }
 
 int32_t sph_neighbor_counts(void* stream, const sph_grid* g, const sph_view* dst,

@@ -44,6 +44,10 @@ constexpr int kQueue = SPH_QUEUE;
 // occupancy target of the momentum scan kernels (waves per SIMD)
 constexpr int kMomWaves = 6;
 constexpr int kUnroll = SPH_UNROLL;
+#ifndef SPH_DENS_UNROLL
+#define SPH_DENS_UNROLL SPH_UNROLL
+#endif
+constexpr int kDensUnroll = SPH_DENS_UNROLL;  // candidates per screen step, density
 __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
 // float4 gather at a non-negative index as a 32-bit byte offset from the
@@ -195,17 +199,18 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
 // cnt <= kQueue - kUnroll on entry.  The exact test (Within32) runs its FP64
 // branch only when a candidate of some lane falls in the +-2e-6 band; with
 // `super` the test is the list-build superset d2 < rl2f instead.
+template <int U = kUnroll>
 __device__ __forceinline__ void screen_block(const View<float>& S, int j0, int j1,
                                              const float4& pd, float r2lo, float r2hi,
                                              double r2d, bool super, float rl2f, int* q,
                                              int& cnt) {
-  float4 c[kUnroll];
+  float4 c[U];
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) c[u] = ld4(S.pos4, j0 + u < j1 ? j0 + u : 0);
-  bool ok[kUnroll], bnd[kUnroll];
+  for (int u = 0; u < U; ++u) c[u] = ld4(S.pos4, j0 + u < j1 ? j0 + u : 0);
+  bool ok[U], bnd[U];
   bool band = false;
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
+  for (int u = 0; u < U; ++u) {
     const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
     const float d2f = dx * dx + dy * dy + dz * dz;
     const bool in = j0 + u < j1;
@@ -220,13 +225,13 @@ __device__ __forceinline__ void screen_block(const View<float>& S, int j0, int j
   }
   if (__any(band)) {
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u)
+    for (int u = 0; u < U; ++u)
       if (bnd[u])
         ok[u] = exact_within((double)c[u].x, (double)c[u].y, (double)c[u].z, (double)pd.x,
                              (double)pd.y, (double)pd.z, r2d);
   }
 #pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
+  for (int u = 0; u < U; ++u) {
     q[cnt * 64] = j0 + u;
     cnt += ok[u] ? 1 : 0;
   }
@@ -344,9 +349,9 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
         int j0 = 0, j1 = 0;
         if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
-          if (__any(cnt > kQueue - kUnroll)) drain();
-          screen_block(S, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q, cnt);
-          j0 += kUnroll;
+          if (__any(cnt > kQueue - kDensUnroll)) drain();
+          screen_block<kDensUnroll>(S, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q, cnt);
+          j0 += kDensUnroll;
         }
       }
     }

@@ -9,7 +9,7 @@ import torch
 
 from .. import _abi
 from .graph import ParticleGraph, combined_array
-from .solver import SurrogateWCSPH, get_user, plain_solver
+from .solver import STATE, SurrogateWCSPH, get_user, plain_solver, set_user
 from .train import dam_scene
 
 
@@ -34,9 +34,15 @@ def compare(model, seed, nsteps=100, n=30000, dt_factor=4, check_every=10, thres
         return f, b, info
 
     out = {"seed": seed, "steps": nsteps, "dt_factor": dt_factor}
-    # reference
+    # reference; both timed runs start warm: a few untimed steps (first-call
+    # allocations, kernel loading) and a restart from the saved state
     f, b, info = scene()
     s = plain_solver(f, b, info)
+    snap = {k: get_user(f, k) for k in STATE}
+    for _ in range(2 * dt_factor):
+        s.integrator.integrate(info["dt"])
+    for k in STATE:
+        set_user(f, k, snap[k])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(nsteps * dt_factor):
@@ -53,6 +59,9 @@ def compare(model, seed, nsteps=100, n=30000, dt_factor=4, check_every=10, thres
     f2, b2, info2 = scene()
     sur = SurrogateWCSPH(f2, b2, info2, model, dt_factor=dt_factor, check_every=check_every,
                          threshold=threshold)
+    st0 = sur.state()
+    sur.run(check_every)
+    sur.reset(st0)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     sur.run(nsteps)

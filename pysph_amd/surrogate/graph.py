@@ -37,7 +37,7 @@ class ParticleGraph:
     """Graph builder over a combined array; `struct` is the sph_mlp_graph of
     the last build."""
 
-    def __init__(self, pa, info, domain=None, cells_per_radius=2, rho_scale=10.0):
+    def __init__(self, pa, info, domain=None, cells_per_radius=2, rho_scale=10.0, max_per=48):
         self.pa = pa
         self.info = dict(info)
         self.device = pa.device
@@ -54,7 +54,14 @@ class ParticleGraph:
         self.off = torch.empty(self.n + 1, dtype=torch.int64, device=self.device)
         self.counts = torch.empty(max(self.n, 1), dtype=torch.int64, device=self.device)
         self.ids = torch.arange(max(self.n, 1), dtype=torch.int64, device=self.device)
-        self.nbr = torch.empty(max(self.n, 1) * 40, dtype=torch.int64, device=self.device)
+        # edge capacity: a bound on the neighbours per node (2D, 2h support,
+        # h = 1.3 dx: ~21 at rest density) with 2x headroom; the CSR is
+        # filled without reading its total back, and an overflow is flagged
+        # on the device and raised at the next check
+        self.max_per = max_per
+        self.cap = max(self.n, 1) * max_per
+        self.nbr = torch.empty(self.cap, dtype=torch.int64, device=self.device)
+        self.overflow = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.struct = _abi.SphMlpGraph()
 
     def build(self):
@@ -64,16 +71,14 @@ class ParticleGraph:
         rmax = 2.0 * cm.max_h
         v = pa.view()
         _abi.call("sph_neighbor_counts", stream, cm.grid, v, v, 2.0, self.counts.data_ptr())
-        total = ctypes.c_int64(0)
         _abi.call("sph_scan_counts", stream, self.n, self.counts.data_ptr(),
-                  self.off.data_ptr(), ctypes.byref(total))
-        ne = int(total.value)
-        if self.nbr.numel() < ne:
-            self.nbr = torch.empty(int(ne * 1.25) + 64, dtype=torch.int64, device=self.device)
-        needed = ctypes.c_int64(0)
-        _abi.call("sph_neighbor_fill", stream, cm.grid, v, v, 2.0, self.ids.data_ptr(),
-                  self.off.data_ptr(), self.nbr.data_ptr(), self.nbr.numel(), ne,
-                  ctypes.byref(needed))
+                  self.off.data_ptr(), None)
+        _abi.call("sph_neighbor_fill_capped", stream, cm.grid, v, v, 2.0, self.ids.data_ptr(),
+                  self.off.data_ptr(), self.nbr.data_ptr(), self.cap,
+                  self.overflow.data_ptr())
+        # the network sizes its edge buffers by the capacity; the exact
+        # count stays on the device (nedges_exact() reads it)
+        ne = self.cap
         # summation density of every node, packed copies for the network
         ov = _ops.packed_override([pa])
         _ops.density(cm, self.prm, pa, [pa], pa.get_carray("rho"), override=ov)
@@ -96,6 +101,16 @@ class ParticleGraph:
         self.nedges = ne
         _ = rmax
         return self
+
+    def nedges_exact(self):
+        """Edges of the last build (synchronises)."""
+        return int(self.off[self.n].item())
+
+    def check_overflow(self):
+        """Raise if a build exceeded the edge capacity (synchronises)."""
+        if int(self.overflow.item()) != 0:
+            raise RuntimeError("surrogate graph: a node has more than %d neighbours; build the "
+                               "ParticleGraph with a larger max_per" % self.max_per)
 
     # ---- user-order access on the device
     def user_to_slot(self, t):

@@ -111,7 +111,18 @@ class SurrogateWCSPH:
         if st is not None:
             self.set_state(st)
         self.graph.build()
-        return get_user(self.graph.pa, "rho")
+        rho = get_user(self.graph.pa, "rho")
+        self.graph.check_overflow()
+        return rho
+
+    def reset(self, st):
+        """Restart from state st (x, y, u, v in user order) with cleared
+        counters."""
+        self.set_state(st)
+        self.t = 0.0
+        self.steps = 0
+        self.plain_left = 0
+        self.stats = {"surrogate": 0, "plain": 0, "checks": 0, "fallbacks": 0, "errors": []}
 
     def density_error(self, a, b):
         """Mean |rho_a - rho_b| / rho0 over the fluid."""

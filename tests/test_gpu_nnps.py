@@ -189,3 +189,45 @@ def test_jump_tolerance_raises():
     pa.set(x=x)
     with pytest.raises(JumpToleranceError):
         cm.update()
+
+
+def test_capped_fill_matches_fill_and_flags_overflow():
+    import torch
+    from pysph_amd import _abi
+    pa = _random_array(4000, 21, "single", h=0.02, dim=2)
+    cm = base.CellManager(arrays_to_bin=[pa])
+    cm.update()
+    n = pa.get_number_of_particles()
+    st = torch.cuda.current_stream().cuda_stream
+    v = pa.view()
+    dev = pa.device
+    counts = torch.empty(n, dtype=torch.int64, device=dev)
+    off = torch.empty(n + 1, dtype=torch.int64, device=dev)
+    ids = torch.arange(n, dtype=torch.int64, device=dev)
+    _abi.call("sph_neighbor_counts", st, cm.grid, v, v, 2.0, counts.data_ptr())
+    total = _abi.ctypes.c_int64(0)
+    _abi.call("sph_scan_counts", st, n, counts.data_ptr(), off.data_ptr(),
+              _abi.ctypes.byref(total))
+    total = int(total.value)
+    ref = torch.empty(total, dtype=torch.int64, device=dev)
+    needed = _abi.ctypes.c_int64(0)
+    _abi.call("sph_neighbor_fill", st, cm.grid, v, v, 2.0, ids.data_ptr(), off.data_ptr(),
+              ref.data_ptr(), total, total, _abi.ctypes.byref(needed))
+    # ample capacity: the same CSR, no flag
+    got = torch.full((total + 64,), -7, dtype=torch.int64, device=dev)
+    flag = torch.zeros(1, dtype=torch.int32, device=dev)
+    _abi.call("sph_neighbor_fill_capped", st, cm.grid, v, v, 2.0, ids.data_ptr(),
+              off.data_ptr(), got.data_ptr(), total + 64, flag.data_ptr())
+    assert int(flag.item()) == 0
+    assert torch.equal(got[:total], ref)
+    assert torch.all(got[total:] == -7)
+    # too small: flagged, nothing written at or past the capacity
+    cap = total // 2
+    got = torch.full((total,), -7, dtype=torch.int64, device=dev)
+    _abi.call("sph_neighbor_fill_capped", st, cm.grid, v, v, 2.0, ids.data_ptr(),
+              off.data_ptr(), got.data_ptr(), cap, flag.data_ptr())
+    assert int(flag.item()) == 1
+    assert torch.all(got[cap:] == -7)
+    o = off.cpu().numpy()
+    full_rows = np.nonzero(o[1:] <= cap)[0]
+    assert torch.equal(got[:int(o[full_rows[-1] + 1])], ref[:int(o[full_rows[-1] + 1])])
