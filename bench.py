@@ -125,10 +125,14 @@ def kernel_names(plan):
         return {"density_fluid": "sph::density_kernel<float, ...>",
                 "stage1": "sph::momentum_kernel<float, 1>",
                 "stage2": "sph::momentum_kernel<float, 2>"}
-    build = "true" if plan.use_list else "false"
-    mom = "momentum_list" if plan.use_list else "momentum_scan"
-    return {"density_fluid": "sph::density_scan<float, true, %s>" % build,
-            "stage1": "sph::%s<float, 1>" % mom, "stage2": "sph::%s<float, 2>" % mom}
+    d = plan.dim
+    if plan.use_list:
+        return {"density_fluid": "sph::density_scan<float, true, true, %d>" % d,
+                "stage1": "sph::momentum_list<float, 1>",
+                "stage2": "sph::momentum_list<float, 2>"}
+    return {"density_fluid": "sph::density_scan<float, true, false, %d>" % d,
+            "stage1": "sph::momentum_scan<float, 1, %d>" % d,
+            "stage2": "sph::momentum_scan<float, 2, %d>" % d}
 
 
 def kernel_bytes(nf, nb):

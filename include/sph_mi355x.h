@@ -391,13 +391,13 @@ typedef struct sph_batch_set {
 } sph_batch_set;
 
 /* Device buffers of a batch, owned by the caller and fixed for the life of
- * the handle (so that a captured step stays valid).  The fluid state
- * alternates between fa (step start/end) and fb (stage 1). */
+ * the handle (so that a captured step stays valid).  The fluid state is in
+ * fa between steps and in fb (the step's binning order) during a step. */
 typedef struct sph_batch_buffers {
     int64_t nf_total, nb_total;
     sph_batch_set fa, fb;        /* fluid sets */
     sph_batch_set bnd, bnd_in;   /* boundary set (binned) and its input order */
-    float *x1a[6], *x1b[6];      /* RK2 stage state X1, before / after its binning */
+    float *x1a[6], *x1b[6];      /* RK2 stage state X1 (x1b: only for which=1 binning) */
     float *k1a[6], *k1b[6];      /* RK2 k1 (momentum 3, XSPH 3), likewise */
     float *fpos4, *fvel4, *faux4;  /* packed fluid copies (float4 per particle) */
     float *bpos4, *bvel4, *baux4;  /* packed boundary copies */
@@ -430,18 +430,25 @@ int64_t sph_batch_ncells(int32_t nscenes, const sph_batch_scene *scenes_host);
 int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene *scenes_host,
                          const sph_batch_buffers *buffers_host, void **handle);
 int32_t sph_batch_destroy(void *handle);
-/* which 0: fluid by X (fa -> fb); 1: fluid by X1 (fb -> fa, x1a -> x1b,
- * k1a -> k1b); 2: boundary (bnd_in -> bnd). */
+/* One RK2 step of every scene is: bin(0), pack(0), density(0, 0),
+ * density(0, 1), stage(1), pack(1), density(1, 0), density(1, 1), stage(2).
+ * The step bins once: stage 2 evaluates X1 in the order of the step's
+ * binning with the stencil widened by the predictor's largest displacement
+ * (recorded on the device by stage 1), the neighbour sets being the same.
+ * which 0: fluid by X (fa -> fb); 1: fluid by X1 (fb -> fa, x1a -> x1b,
+ * k1a -> k1b; not part of the step); 2: boundary (bnd_in -> bnd). */
 int32_t sph_batch_bin(void *handle, void *stream, int32_t which);
-/* which 0: fluid from fb; 1: fluid from X1 (x1b, components >= dim and h, m
- * from fa); 2: boundary. */
+/* which 0: fluid from fb; 1: fluid from X1 (x1a, components >= dim and h, m
+ * from fb); 2: boundary. */
 int32_t sph_batch_pack(void *handle, void *stream, int32_t which);
-/* SPHRho + TaitEquation of every scene: phase 0 on fb (stage 1), phase 1
- * on fa with X1 positions (stage 2); set 0 fluid destinations, 1 boundary. */
+/* SPHRho + TaitEquation of every scene on fb's order: phase 0 at X (stage
+ * 1, results in fb), phase 1 at X1 (stage 2, results in fa); set 0 fluid
+ * destinations, 1 boundary. */
 int32_t sph_batch_density(void *handle, void *stream, int32_t phase, int32_t set,
                           const sph_batch_sched *sched);
-/* RK2 stage of every scene's fluid: 1 = predictor (fb -> x1a, k1a),
- * 2 = final update of fa in place. */
+/* RK2 stage of every scene's fluid: 1 = predictor (fb -> x1a, k1a, and the
+ * largest displacement), 2 = final update (fb, x1a, k1a -> fa, with h, m, uid
+ * copied along, so that fa holds the whole state for the next step). */
 int32_t sph_batch_stage(void *handle, void *stream, int32_t stage,
                         const sph_batch_sched *sched);
 

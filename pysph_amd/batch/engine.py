@@ -232,7 +232,6 @@ class BatchedWCSPH:
         call("sph_batch_density", h, stream, 0, 0, ctypes.byref(sd))
         call("sph_batch_density", h, stream, 0, 1, ctypes.byref(sd))
         call("sph_batch_stage", h, stream, 1, ctypes.byref(ss))
-        call("sph_batch_bin", h, stream, 1)
         call("sph_batch_pack", h, stream, 1)
         call("sph_batch_density", h, stream, 1, 0, ctypes.byref(sd))
         call("sph_batch_density", h, stream, 1, 1, ctypes.byref(sd))

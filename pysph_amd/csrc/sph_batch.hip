@@ -1,9 +1,10 @@
 // Batched multi-scene WCSPH: hundreds of independent scenes of uneven size
 // stepped together with one launch per phase.
 //
-// Layout: all scenes' fluid particles form one concatenated set (two copies,
-// fa and fb, that the two binnings of an RK2 step alternate between), all
-// boundary particles another.  Scene s owns the cells [cell_base_s,
+// Layout: all scenes' fluid particles form one concatenated set (two copies:
+// the binning at the start of an RK2 step permutes fa into fb, the step
+// evaluates both stages in that order and writes the final state back into
+// fa), all boundary particles another.  Scene s owns the cells [cell_base_s,
 // cell_base_s + cells_s) of one global cell space, so one counting sort bins
 // every scene and keeps each scene contiguous and cell-ordered.
 //
@@ -24,7 +25,7 @@ namespace sph {
 
 struct SceneArgs {
   Grid g;
-  PassArgs<float> dens[2][2];  // [phase][0 fluid dst, 1 boundary dst]
+  PassArgs<float> dens[2][2];  // [stage - 1][0 fluid dst, 1 boundary dst]
   PassArgs<float> stage[2];    // [stage - 1]
 };
 
@@ -42,6 +43,7 @@ struct Batch {
   int* d_seg[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};     // 9 segment starts
   int ntiles[2] = {0, 0};
   int* d_counters = nullptr;  // kXcds per launch site
+  float* d_disp = nullptr;    // largest predictor displacement of the step
 };
 
 constexpr int kSites = 6;
@@ -110,6 +112,18 @@ void batch_stage_kernel(
     momentum_scan_body<float, MODE, DIM>(A.g, A.stage[MODE - 1],
                                          __builtin_amdgcn_readfirstlane(T.start) + lane, q);
   });
+}
+
+// Three 32-bit arrays copied element-wise (dst[i] = src[i]).
+__global__ __launch_bounds__(kBlock) void copy3_kernel(
+    int64_t n, const uint32_t* __restrict__ a0, uint32_t* __restrict__ b0,
+    const uint32_t* __restrict__ a1, uint32_t* __restrict__ b1, const uint32_t* __restrict__ a2,
+    uint32_t* __restrict__ b2) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  b0[i] = a0[i];
+  b1[i] = a1[i];
+  b2[i] = a2[i];
 }
 
 // pos4 = (x, y, z, h), vel4 = (u, v, w, m) from separate component arrays.
@@ -183,6 +197,7 @@ static void free_batch(Batch* b) {
       (void)hipFree(b->d_seg[s][o]);
     }
   (void)hipFree(b->d_counters);
+  (void)hipFree(b->d_disp);
   delete b;
 }
 
@@ -341,6 +356,10 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
   b->ncells = ncells;
   b->buf = B;
 
+  if (upload(&b->d_disp, std::vector<float>(1, 0.f)) != SPH_OK) {
+    free_batch(b);
+    return SPH_EINVAL;
+  }
   std::vector<SceneArgs> args(nscenes);
   std::vector<BatchBinGrid> bgrid(nscenes);
   int64_t base = 0;
@@ -359,8 +378,11 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
     const int32_t* fcs = B.fcell_start + base;
     const int32_t* bcs = S.nb > 0 ? B.bcell_start + base : nullptr;
     base += scene_cells(S);
-    // the fluid set of each phase: fb during stage 1, fa during stage 2
-    const sph_batch_set* fset[2] = {&B.fb, &B.fa};
+    // both stages evaluate the fluid in the order of the step's binning
+    // (fb); stage 2 reads the predictor X1 through the packed copies and
+    // widens its stencil by the predictor's displacement, and the step's
+    // results (density of X1, final state) go to fa
+    const sph_batch_set* fset[2] = {&B.fb, &B.fb};
     for (int ph = 0; ph < 2; ++ph) {
       const View<float> fsrc = set_view(*fset[ph], 0, B.nf_total, B.fpos4, B.fvel4, B.faux4,
                                         B.fcell, fcs);
@@ -381,11 +403,12 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
           a.src[1] = bsrc;
           a.flags[1] = SPH_SRC_MOMENTUM | SPH_SRC_XSPH;
         }
-        const sph_batch_set& D = set == 0 ? *fset[ph] : B.bnd;
+        const sph_batch_set& D = set == 0 ? (ph == 0 ? B.fb : B.fa) : B.bnd;
         const int64_t off = set == 0 ? S.f0 : S.b0;
         a.rho = D.rho + off;
         a.p = D.p + off;
         a.cs = D.cs + off;
+        if (ph == 1) a.disp = b->d_disp;
       }
       // stage ph+1: momentum of the fluid
       PassArgs<float>& m = A.stage[ph];
@@ -402,14 +425,19 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
       m.do_mom = 1;
       m.do_xsph = S.prm.has_xsph ? 1 : 0;
       m.dt = S.dt;
-      const sph_batch_set& X = *fset[ph];
+      const sph_batch_set& X = B.fb;
       const float* xs[6] = {X.x, X.y, X.z, X.u, X.v, X.w};
+      float* fin[6] = {B.fa.x, B.fa.y, B.fa.z, B.fa.u, B.fa.v, B.fa.w};
       for (int c = 0; c < 6; ++c) {
         const bool need = (c % 3) < dim;
         m.s0[c] = need ? xs[c] + S.f0 : nullptr;
-        m.k1[c] = need ? (ph == 0 ? B.k1a[c] : B.k1b[c]) + S.f0 : nullptr;
-        m.out[c] = need ? (ph == 0 ? B.x1a[c] : const_cast<float*>(xs[c])) + S.f0 : nullptr;
+        m.k1[c] = need ? B.k1a[c] + S.f0 : nullptr;
+        m.out[c] = need ? (ph == 0 ? B.x1a[c] : fin[c]) + S.f0 : nullptr;
       }
+      if (ph == 0)
+        m.disp_out = b->d_disp;  // stage 1 records the predictor's displacement
+      else
+        m.disp = b->d_disp;      // stage 2 widens its stencil by it
     }
   }
   int32_t rc = upload(&b->d_args, args);
@@ -498,17 +526,17 @@ int32_t sph_batch_pack(void* handle, void* stream, int32_t which) {
   const sph_batch_buffers& B = b->buf;
   const int64_t n = which == 2 ? b->nb : b->nf;
   if (n == 0) return SPH_OK;
-  const sph_batch_set& S = which == 0 ? B.fb : (which == 1 ? B.fa : B.bnd);
+  const sph_batch_set& S = which == 2 ? B.bnd : B.fb;
   const float *x = S.x, *y = S.y, *z = S.z, *u = S.u, *v = S.v, *w = S.w;
-  if (which == 1) {  // X1 with components >= dim from X
+  if (which == 1) {  // X1 (in fb's order) with components >= dim from X
     const int dim = b->dim;
-    x = B.x1b[0];
-    y = B.x1b[1];
-    u = B.x1b[3];
-    v = B.x1b[4];
+    x = B.x1a[0];
+    y = B.x1a[1];
+    u = B.x1a[3];
+    v = B.x1a[4];
     if (dim > 2) {
-      z = B.x1b[2];
-      w = B.x1b[5];
+      z = B.x1a[2];
+      w = B.x1a[5];
     }
   }
   float* pos4 = which == 2 ? B.bpos4 : B.fpos4;
@@ -557,6 +585,7 @@ int32_t sph_batch_stage(void* handle, void* stream, int32_t stage, const sph_bat
   const int nbk = launch_blocks(b, sched, nt);
   const Tile* tl = b->d_tiles[0][sched->order];
   const int* sg = b->d_seg[0][sched->order];
+  if (stage == 1) SPH_TRY(fill_i32(st, (int32_t*)b->d_disp, 1, 0));
 #define SPH_STAGE(M, D)                                                                   \
   hipLaunchKernelGGL((batch_stage_kernel<M, D>), dim3(nbk), dim3(kBlock), 0, st, b->d_args, tl, \
                      nt, sg, sched->mode, sched->grab, ctr)
@@ -568,6 +597,14 @@ int32_t sph_batch_stage(void* handle, void* stream, int32_t stage, const sph_bat
     else SPH_STAGE(2, 2);
   }
 #undef SPH_STAGE
+  if (stage == 2) {
+    // the step's final state in fa also needs the fields the stages do not
+    // write (h, m, uid) in the same order
+    const sph_batch_buffers& B = b->buf;
+    hipLaunchKernelGGL(copy3_kernel, dim3(nblocks_for(b->nf)), dim3(kBlock), 0, st, b->nf,
+                       (const uint32_t*)B.fb.h, (uint32_t*)B.fa.h, (const uint32_t*)B.fb.m,
+                       (uint32_t*)B.fa.m, (const uint32_t*)B.fb.uid, (uint32_t*)B.fa.uid);
+  }
   return launch_status("sph_batch_stage");
 }
 

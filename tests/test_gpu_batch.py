@@ -38,8 +38,6 @@ def _plain(f, b, info, nsteps, cells_per_radius=2):
     plan = s.integrator.plan
     assert plan is not None and plan.use_packed
     plan.set_cells_per_radius(cells_per_radius)
-    # the batched engine re-bins by X1 before stage 2
-    plan.stage2_rebin = True
     for _ in range(nsteps):
         s.integrator.integrate(info["dt"])
     return {c: f.get(c) for c in FIELDS}, (None if b is None else b.get("rho"))
