@@ -50,6 +50,39 @@ def build():
 
 
 _lib = None
+_REF_DIR = "/root/reference/bench/nnps_c++"
+REF_NNPS2D = os.path.join(_HERE, "_ref", "nnps2d")
+
+
+def build_ref():
+    """Compile the reference's 2D C++ neighbour search into _ref/ when the
+    reference sources are present (returns whether the binary exists)."""
+    if os.path.isdir(_REF_DIR):
+        subprocess.check_call(["make", "-s", "-C", _HERE, "ref"])
+    return os.path.exists(REF_NNPS2D)
+
+
+def ref_nnps2d(x, y, radius):
+    """Neighbour sets of every point of (x, y) within `radius` from the
+    reference's bench/nnps_c++ search: list of sorted int64 arrays."""
+    import tempfile
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    with tempfile.TemporaryDirectory() as d:
+        fin, fout = os.path.join(d, "in.bin"), os.path.join(d, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.int64(len(x)).tobytes())
+            f.write(np.float64(radius).tobytes())
+            f.write(x.tobytes())
+            f.write(y.tobytes())
+        subprocess.check_call([REF_NNPS2D, fin, fout])
+        raw = np.fromfile(fout, dtype=np.int64)
+    rows, k = [], 0
+    for _ in range(len(x)):
+        c = int(raw[k])
+        rows.append(np.sort(raw[k + 1:k + 1 + c]))
+        k += 1 + c
+    return rows
 
 
 def lib():

@@ -231,3 +231,24 @@ def test_capped_fill_matches_fill_and_flags_overflow():
     o = off.cpu().numpy()
     full_rows = np.nonzero(o[1:] <= cap)[0]
     assert torch.equal(got[:int(o[full_rows[-1] + 1])], ref[:int(o[full_rows[-1] + 1])])
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(O.REF_NNPS2D),
+                    reason="reference C++ neighbour search not built")
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_2d_sets_match_reference_cpp_nnps(precision):
+    """HIP sets against the reference's bench/nnps_c++ search run on the
+    same (precision-rounded) points, uniform h, radius 2h."""
+    rng = np.random.Generator(np.random.PCG64(23))
+    dt = np.float32 if precision == "single" else np.float64
+    x = rng.uniform(0, 1, 8000).astype(dt).astype(np.float64)
+    y = rng.uniform(0, 1, 8000).astype(dt).astype(np.float64)
+    h = 0.0125
+    pa = base.get_particle_array(cl_precision=precision, name="r2d", x=x, y=y,
+                                 h=np.full(len(x), h))
+    (off, nb), _ = _gpu_csr(pa, pa)
+    ref = O.ref_nnps2d(x, y, 2.0 * h)
+    assert np.array_equal(np.diff(off), [len(r) for r in ref]), "neighbour counts differ"
+    for a, b in zip(csr_sorted_rows(off, nb), ref):
+        assert np.array_equal(a, b)
+    assert off[-1] > 5 * len(x)
