@@ -107,6 +107,11 @@ class WCSPHPlan:
         # evaluated per step)
         self.cache_static = False
         self._caches = {}
+        # concurrent_density: the density passes of the arrays other than
+        # the moving one run on a second stream (default priority), beside
+        # the moving array's pass; the stage kernel waits for both
+        self.concurrent_density = True
+        self._side = None
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
 
@@ -336,19 +341,31 @@ class WCSPHPlan:
 
     def _density(self, disp=None, override=None, list_mode=0):
         prm = self.prm
+        main = torch.cuda.current_stream()
+        side = None
+        if self.concurrent_density and len(self.rho_calcs) > 1:
+            if self._side is None:
+                self._side = torch.cuda.Stream(device=self.fluid.device)
+            side = self._side
+            # the side stream starts after everything queued so far
+            side.wait_stream(main)
         for c in self.rho_calcs:
             pa = c.dest
             rho = pa.get_carray("rho")
             # only the moving array has a momentum pass that reads a list
             nl = self._nlist(pa, list_mode) if pa is self.fluid else None
             cache = self._src_cache(c) if nl is None else None
+            args = (self.cm, prm, pa, c.sources, rho)
             if id(pa) in self.tait_arrays:
-                _ops.density(self.cm, prm, pa, c.sources, rho, pa.get_carray("p"),
-                             pa.get_carray("cs"), disp=disp, override=override, nl=nl,
-                             cache=cache)
+                args += (pa.get_carray("p"), pa.get_carray("cs"))
+            if side is not None and pa is not self.fluid:
+                with torch.cuda.stream(side):
+                    _ops.density(*args, disp=disp, override=override, nl=nl, cache=cache)
             else:
-                _ops.density(self.cm, prm, pa, c.sources, rho, disp=disp, override=override,
-                             nl=nl, cache=cache)
+                _ops.density(*args, disp=disp, override=override, nl=nl, cache=cache)
+        if side is not None:
+            # the momentum pass reads every array's density
+            main.wait_stream(side)
 
     def _stage(self, stage, dt, b, override=None, disp_in=None, disp_out=None):
         pa = self.fluid
