@@ -67,8 +67,9 @@ typedef struct sph_view {
     /* Optional packed FP32 copies (float4 per particle) read by the fast
      * neighbour passes: pos4 = (x,y,z,h), vel4 = (u,v,w,m) from sph_pack,
      * aux4 = (rho, p, cs, p/rho^2) written by sph_density.  When pos4 and
-     * vel4 are set on the destination and every source (FP32 storage), the
-     * passes read them instead of the SoA fields. */
+     * vel4 are set on the destination and every source (FP32 storage, at
+     * most 2^27 particles each), the passes read them instead of the SoA
+     * fields; larger arrays run the SoA kernels. */
     const void *pos4, *vel4, *aux4;
 } sph_view;
 

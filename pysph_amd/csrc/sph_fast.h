@@ -52,7 +52,7 @@ __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
 // float4 gather at a non-negative index as a 32-bit byte offset from the
 // array base, so the load takes the scalar-base + 32-bit-offset form
-// (arrays stay below 2^28 particles).
+// (packed arrays hold at most kPackedMaxN = 2^27 particles).
 __device__ __forceinline__ float4 ld4(const float4* __restrict__ base, int j) {
   return *(const float4*)((const char*)base + (unsigned)j * 16u);
 }
@@ -190,30 +190,51 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   j1 = cstart[base + x1 + 1];
 }
 
+// Raw buffer resource over a source's packed positions (n * 16 bytes,
+// gfx9 resource word 3): a load at or past the end returns zeros instead of
+// touching memory, so candidate loads need no index clamp.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos4_rsrc(const View<float>& S) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)S.pos4, (short)0,
+                                           (int)((unsigned)S.n * 16u), 0x00020000);
+}
+
+// (x, y, z) of candidate slot u of a block starting at byte offset off.
+__device__ __forceinline__ float3 ld_xyz(__amdgpu_buffer_rsrc_t rs, int off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b96(rs, off, 0, 0);
+  return make_float3(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]));
+}
+
 // Screen the candidates j0 .. j0+kUnroll-1 (those below j1) of one stencil
 // row against destination pd and push the accepted ones into the lane's
 // LDS queue q (entries 64 apart), branch-free: every lane loads kUnroll
-// candidates (index 0 stands in past the row end; the source is not empty
-// while any lane has candidates), writes every candidate into the next
-// queue slot and advances its count only on acceptance.  The caller keeps
+// candidates through the bounds-checked buffer view (one base offset,
+// the slots as immediate offsets; slots past the row end are loaded but
+// never accepted), writes every candidate into the next queue slot and
+// advances its count only on acceptance.  The caller keeps
 // cnt <= kQueue - kUnroll on entry.  The exact test (Within32) runs its FP64
 // branch only when a candidate of some lane falls in the +-2e-6 band; with
 // `super` the test is the list-build superset d2 < rl2f instead.
 template <int U = kUnroll>
-__device__ __forceinline__ void screen_block(const View<float>& S, int j0, int j1,
-                                             const float4& pd, float r2lo, float r2hi,
-                                             double r2d, bool super, float rl2f, int* q,
-                                             int& cnt) {
-  float4 c[U];
+__device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buffer_rsrc_t rs,
+                                             int j0, int j1, const float4& pd, float r2lo,
+                                             float r2hi, double r2d, bool super, float rl2f,
+                                             int* q, int& cnt) {
+  float3 c[U];
+  int off = (int)((unsigned)j0 * 16u);
+  // keep off opaque, so that the slots become immediate load offsets
+  // instead of one (j0 + u) * 16 per slot (an empty register constraint:
+  // no instruction is emitted)
+  asm("" : "+v"(off));
 #pragma unroll
-  for (int u = 0; u < U; ++u) c[u] = ld4(S.pos4, j0 + u < j1 ? j0 + u : 0);
+  for (int u = 0; u < U; ++u) c[u] = ld_xyz(rs, off + 16 * u);
+  const int rem = j1 - j0;
   bool ok[U], bnd[U];
   bool band = false;
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
     const float d2f = dx * dx + dy * dy + dz * dz;
-    const bool in = j0 + u < j1;
+    const bool in = u < rem;
     if (super) {
       ok[u] = in && d2f < rl2f;
       bnd[u] = false;
@@ -318,6 +339,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
   int lpos = 0;
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
+    const __amdgpu_buffer_rsrc_t rs = pos4_rsrc(S);
     CT nr = CT(0);
     if (!BUILD && s == a.cache_src && a.cache_mode == 2) {
       // stored partial sum of a static source (exact round trip via double)
@@ -350,7 +372,8 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
         if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
           if (__any(cnt > kQueue - kDensUnroll)) drain();
-          screen_block<kDensUnroll>(S, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q, cnt);
+          screen_block<kDensUnroll>(S, rs, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q,
+                                    cnt);
           j0 += kDensUnroll;
         }
       }
@@ -509,6 +532,7 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
   CT dv[3] = {CT(0), CT(0), CT(0)};
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
+    const __amdgpu_buffer_rsrc_t rs = pos4_rsrc(S);
     const bool do_m = (a.flags[s] & SPH_SRC_MOMENTUM) != 0 && a.do_mom;
     const bool do_x = (a.flags[s] & SPH_SRC_XSPH) != 0 && a.do_xsph;
     if (!do_m && !do_x) continue;
@@ -534,7 +558,7 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
         if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
           if (__any(cnt > kQueue - kUnroll)) drain();
-          screen_block(S, j0, j1, D.pd, r2lo, r2hi, r2d, false, 0.f, q, cnt);
+          screen_block(S, rs, j0, j1, D.pd, r2lo, r2hi, r2d, false, 0.f, q, cnt);
           j0 += kUnroll;
         }
       }

@@ -161,13 +161,21 @@ __device__ __forceinline__ void tait(double rho, double co, double ro, double ga
   cs = pow(ratio, gamma2) * co;
 }
 
-// True when the destination and every source carry packed FP32 inputs.
+// Largest array the packed fast path serves: its gathers address a float4
+// array with 32-bit byte offsets.
+constexpr int kPackedMaxN = 1 << 27;
+
+// True when the destination and every source carry packed FP32 inputs (and
+// are small enough for 32-bit byte offsets into them).
 template <typename T>
 static inline bool packed_inputs(const PassArgs<T>& a, bool need_aux) {
   if (sizeof(T) != 4) return false;
   if (!a.dst.pos4 || !a.dst.vel4 || (need_aux && !a.dst.aux4)) return false;
-  for (int s = 0; s < a.nsrc; ++s)
+  if (a.dst.n > kPackedMaxN) return false;
+  for (int s = 0; s < a.nsrc; ++s) {
     if (!a.src[s].pos4 || !a.src[s].vel4 || (need_aux && !a.src[s].aux4)) return false;
+    if (a.src[s].n > kPackedMaxN) return false;
+  }
   return true;
 }
 
