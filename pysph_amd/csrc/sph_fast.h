@@ -57,6 +57,12 @@ __device__ __forceinline__ float4 ld4(const float4* __restrict__ base, int j) {
   return *(const float4*)((const char*)base + (unsigned)j * 16u);
 }
 
+// compile-time flag for generic lambdas
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
@@ -396,10 +402,13 @@ struct MomDst {
   int dim;
 };
 
-template <typename CT>
-__device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, const float4& vs,
-                                         const float4& as, bool do_m, bool do_x, CT* am,
-                                         CT* ax) {
+// One pair of the momentum (DM) and XSPH (DX) sums.  The roles are fixed
+// at compile time, so no branch separates a pair's gathers: the compiler
+// would otherwise sink the loads that only one role uses behind that
+// role's branch, and wait for memory twice per pair.
+template <typename CT, bool DM, bool DX>
+__device__ __forceinline__ void mom_pair_r(const MomDst<CT>& D, const float4& ps,
+                                           const float4& vs, const float4& as, CT* am, CT* ax) {
   CT rx, ry, rz;
   const CT d2 = pair_d2<CT>(D.pd, ps, rx, ry, rz);
   const CT hab = CT(0.5) * (D.ha + (CT)ps.w);
@@ -416,7 +425,7 @@ __device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, 
   const CT vx = (CT)D.vd.x - (CT)vs.x, vy = (CT)D.vd.y - (CT)vs.y, vz = (CT)D.vd.z - (CT)vs.z;
   const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
   const CT cab = CT(0.5) * (D.ca + (CT)as.z);
-  if (do_m) {
+  if (DM) {
     const CT dot = vx * rx + vy * ry + vz * rz;
     const CT mu = hab * dot * frcp<CT>(d2 + D.eta2 * hab * hab);
     const CT piab = (-D.alpha * cab * mu + D.beta * mu * mu) * irhoab;
@@ -426,12 +435,25 @@ __device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, 
     am[1] += gf * ry;
     am[2] += gf * rz;
   }
-  if (do_x) {
+  if (DX) {
     const CT temp = mb * (w_sel<CT>(qq) * fac) * irhoab;
     ax[0] += temp * -vx;
     ax[1] += temp * -vy;
     ax[2] += temp * -vz;
   }
+}
+
+// The same pair with the roles chosen at run time (list passes).
+template <typename CT>
+__device__ __forceinline__ void mom_pair(const MomDst<CT>& D, const float4& ps, const float4& vs,
+                                         const float4& as, bool do_m, bool do_x, CT* am,
+                                         CT* ax) {
+  if (do_m && do_x)
+    mom_pair_r<CT, true, true>(D, ps, vs, as, am, ax);
+  else if (do_m)
+    mom_pair_r<CT, true, false>(D, ps, vs, as, am, ax);
+  else if (do_x)
+    mom_pair_r<CT, false, true>(D, ps, vs, as, am, ax);
 }
 
 template <typename CT, int DIM = 0>
@@ -539,14 +561,24 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
     CT am[3] = {CT(0), CT(0), CT(0)};
     CT ax[3] = {CT(0), CT(0), CT(0)};
     int cnt = 0;
-    auto drain = [&]() {
+    // one drain loop per role set (the roles are uniform over a source)
+    auto drain_r = [&](auto dm, auto dx) {
       for (int k = 0; __any(k < cnt); ++k) {
         if (k < cnt) {
           const int j = q[k * 64];
-          mom_pair<CT>(D, ld4(S.pos4, j), ld4(S.vel4, j), ld4(S.aux4, j), do_m, do_x, am, ax);
+          mom_pair_r<CT, decltype(dm)::value, decltype(dx)::value>(
+              D, ld4(S.pos4, j), ld4(S.vel4, j), ld4(S.aux4, j), am, ax);
         }
       }
       cnt = 0;
+    };
+    auto drain = [&]() {
+      if (do_m && do_x)
+        drain_r(BoolC<true>{}, BoolC<true>{});
+      else if (do_m)
+        drain_r(BoolC<true>{}, BoolC<false>{});
+      else
+        drain_r(BoolC<false>{}, BoolC<true>{});
     };
     for (int dz = -ring; dz <= ring; ++dz) {
       const bool zok = local && dz >= dzlo && dz <= dzhi;
