@@ -807,7 +807,11 @@ def main():
         "valu": {"kernel": key, "flops_per_launch": flops_dom,
                  "achieved_tflops": flops_dom / (dom_ms * 1e-3) / 1e12,
                  "peak_tflops": FP32_PEAK_TFLOPS,
-                 "frac": flops_dom / (dom_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS},
+                 "frac": flops_dom / (dom_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                 # the kernels are built without packed FP32 instructions:
+                 # one FP32 FMA per lane and cycle is half the published peak
+                 "peak_tflops_unpacked": FP32_PEAK_TFLOPS / 2,
+                 "frac_unpacked": flops_dom / (dom_ms * 1e-3) / 1e12 / (FP32_PEAK_TFLOPS / 2)},
         # per-launch events inside the step loop (each event pair adds its
         # own cost to the launch it brackets)
         "kernel_ms_events_in_loop": {k: float(np.mean(v)) for k, v in kt.items()},
