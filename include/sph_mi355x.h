@@ -66,7 +66,7 @@ typedef struct sph_view {
     const int32_t *cell_start; /* ncells + 1 offsets from sph_bin */
     /* Optional packed FP32 copies (float4 per particle) read by the fast
      * neighbour passes: pos4 = (x,y,z,h), vel4 = (u,v,w,m) from sph_pack,
-     * aux4 = (rho, p, cs, p/rho^2) written by sph_density.  When pos4 and
+     * aux4 = (rho, cs, p/rho^2, p) written by sph_density.  When pos4 and
      * vel4 are set on the destination and every source (FP32 storage, at
      * most 2^27 particles each), the passes read them instead of the SoA
      * fields; larger arrays run the SoA kernels. */
@@ -245,7 +245,7 @@ typedef struct sph_nlist {
 /* SPHRho over all sources (density_funcs.pyx:38-84), optionally followed by
  * TaitEquation on the destination (eos_funcs.pyx:105-116) when p_dev and
  * cs_dev are both non-NULL.  With packed inputs the destination's aux4
- * (when set) receives (rho, p, cs, p/rho^2).  `disp_dev` (nullable, one float) is the largest
+ * (when set) receives (rho, cs, p/rho^2, p).  `disp_dev` (nullable, one float) is the largest
  * coordinate displacement of any particle since the arrays were binned; the
  * stencil ring grows to cover it so that a binning can be reused across an
  * RK2 stage.  rmax = the largest search radius (radius_scale * max h). */

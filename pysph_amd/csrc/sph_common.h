@@ -196,7 +196,7 @@ struct View {
   const int* cstart;
   const float4* pos4;  // packed (x,y,z,h), FP32 storage only
   const float4* vel4;  // packed (u,v,w,m)
-  float4* aux4;        // packed (rho,p,cs,p/rho^2)
+  float4* aux4;        // packed (rho,cs,p/rho^2,p)
 };
 
 template <typename T>

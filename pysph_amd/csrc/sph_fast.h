@@ -5,7 +5,8 @@
 // Packed FP32 fast path of the neighbour passes.
 //
 // Inputs are float4 copies per particle: pos4 = (x,y,z,h), vel4 = (u,v,w,m),
-// aux4 = (rho,p,cs,p/rho^2).  One lane per destination particle, the
+// aux4 = (rho,cs,p/rho^2,p): the FP32 pair arithmetic reads the first three,
+// one 12-byte load.  One lane per destination particle, the
 // particles sorted by cell so that the lanes of a wave share stencil rows.
 //
 // Cell scans: a lane screens the candidates of its stencil rows (one 16-byte
@@ -305,8 +306,8 @@ __device__ __forceinline__ void density_epilogue(const PassArgs<float>& a, int i
     a.p[i] = pf;
     a.cs[i] = (float)cs;
     if (a.dst.aux4 != nullptr)
-      a.dst.aux4[i] = make_float4(rhof, pf, (float)cs,
-                                  (float)((double)pf / ((double)rhof * (double)rhof)));
+      a.dst.aux4[i] = make_float4(rhof, (float)cs,
+                                  (float)((double)pf / ((double)rhof * (double)rhof)), pf);
   }
 }
 
@@ -423,8 +424,8 @@ __device__ __forceinline__ void mom_pair_r(const MomDst<CT>& D, const float4& ps
   // exactly), and every source field is read before the branches, so the
   // three gathers of a pair go out together
   const CT vx = (CT)D.vd.x - (CT)vs.x, vy = (CT)D.vd.y - (CT)vs.y, vz = (CT)D.vd.z - (CT)vs.z;
-  const CT Ab = sizeof(CT) == 8 ? (CT)as.y / (rhob * rhob) : (CT)as.w;
-  const CT cab = CT(0.5) * (D.ca + (CT)as.z);
+  const CT Ab = sizeof(CT) == 8 ? (CT)as.w / (rhob * rhob) : (CT)as.z;
+  const CT cab = CT(0.5) * (D.ca + (CT)as.y);
   if (DM) {
     const CT dot = vx * rx + vy * ry + vz * rz;
     const CT mu = hab * dot * frcp<CT>(d2 + D.eta2 * hab * hab);
@@ -466,8 +467,8 @@ __device__ __forceinline__ MomDst<CT> mom_dst(const PassArgs<float>& a, int i, b
   D.dim = DIM ? DIM : a.dim;
   D.ha = (CT)D.pd.w;
   D.rhoa = (CT)ad.x;
-  D.ca = (CT)ad.z;
-  D.Aa = sizeof(CT) == 8 ? (CT)ad.y / (D.rhoa * D.rhoa) : (CT)ad.w;
+  D.ca = (CT)ad.y;
+  D.Aa = sizeof(CT) == 8 ? (CT)ad.w / (D.rhoa * D.rhoa) : (CT)ad.z;
   D.alpha = (CT)a.alpha;
   D.beta = (CT)a.beta;
   D.eta2 = (CT)(a.eta * a.eta);

@@ -231,7 +231,7 @@ class WCSPHPlan:
 
     # ------------------------------------------------------ packed copies
     # FP32 storage: every array carries float4 copies (x,y,z,h), (u,v,w,m)
-    # and (rho,p,cs,p/rho^2) in its current slot order, read by the fast
+    # and (rho,cs,p/rho^2,p) in its current slot order, read by the fast
     # neighbour passes.  pos4/vel4 are re-packed after every binning of the
     # array (and from X1 after the stage-2 binning); aux4 is written by the
     # density pass that precedes every momentum pass.
