@@ -71,6 +71,10 @@ typedef struct sph_view {
      * most 2^27 particles each), the passes read them instead of the SoA
      * fields; larger arrays run the SoA kernels. */
     const void *pos4, *vel4, *aux4;
+    /* Optional packed (x,y,z) FP32 copy, 12 bytes per particle, from
+     * sph_pack: when a source carries it, the candidate screen of the fast
+     * passes reads four candidates in three 16-byte loads. */
+    const void *pos3;
 } sph_view;
 
 /* The cell grid shared by all arrays of one Particles set.  Cell (i,j,k) of
@@ -165,9 +169,11 @@ int32_t sph_periodic_wrap(void *stream, const sph_grid *g, int32_t dtype, int64_
                           void *const *pos_dev_host, int32_t nextra,
                           void *const *extra_dev_host);
 
-/* Write the packed copies pos4[i] = (x,y,z,h) and vel4[i] = (u,v,w,m) of an
- * FP32 array (float4 each; either may be NULL). */
-int32_t sph_pack(void *stream, const sph_view *a, void *pos4_dev, void *vel4_dev);
+/* Write the packed copies pos4[i] = (x,y,z,h), vel4[i] = (u,v,w,m) (float4
+ * each) and pos3[i] = (x,y,z) (3 floats) of an FP32 array; any of the
+ * three may be NULL. */
+int32_t sph_pack(void *stream, const sph_view *a, void *pos4_dev, void *vel4_dev,
+                 void *pos3_dev);
 
 /* ---- FixedDestNbrParticleLocator.get_nearest_particles (nnps.pyx:458-620)
  *      as a CSR export: the exact FP64 test

@@ -197,6 +197,7 @@ struct View {
   const float4* pos4;  // packed (x,y,z,h), FP32 storage only
   const float4* vel4;  // packed (u,v,w,m)
   float4* aux4;        // packed (rho,cs,p/rho^2,p)
+  const float* pos3;   // packed (x,y,z), 3 floats per particle
 };
 
 template <typename T>
@@ -211,6 +212,7 @@ inline View<T> to_view(const sph_view& a) {
   v.pos4 = (const float4*)a.pos4;
   v.vel4 = (const float4*)a.vel4;
   v.aux4 = (float4*)a.aux4;
+  v.pos3 = (const float*)a.pos3;
   return v;
 }
 

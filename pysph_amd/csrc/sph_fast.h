@@ -197,10 +197,14 @@ __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restr
   j1 = cstart[base + x1 + 1];
 }
 
-// Raw buffer resource over a source's packed positions (n * 16 bytes,
-// gfx9 resource word 3): a load at or past the end returns zeros instead of
+// Raw buffer resource over a source's packed positions, the (x,y,z) copy
+// when the source has one (n * 12 bytes), else pos4 (n * 16 bytes); gfx9
+// resource word 3.  A load at or past the end returns zeros instead of
 // touching memory, so candidate loads need no index clamp.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t pos4_rsrc(const View<float>& S) {
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t cand_rsrc(const View<float>& S) {
+  if (S.pos3 != nullptr)
+    return __builtin_amdgcn_make_buffer_rsrc((void*)S.pos3, (short)0,
+                                             (int)((unsigned)S.n * 12u), 0x00020000);
   return __builtin_amdgcn_make_buffer_rsrc((void*)S.pos4, (short)0,
                                            (int)((unsigned)S.n * 16u), 0x00020000);
 }
@@ -214,9 +218,11 @@ __device__ __forceinline__ float3 ld_xyz(__amdgpu_buffer_rsrc_t rs, int off) {
 // Screen the candidates j0 .. j0+kUnroll-1 (those below j1) of one stencil
 // row against destination pd and push the accepted ones into the lane's
 // LDS queue q (entries 64 apart), branch-free: every lane loads kUnroll
-// candidates through the bounds-checked buffer view (one base offset,
-// the slots as immediate offsets; slots past the row end are loaded but
-// never accepted), writes every candidate into the next queue slot and
+// candidates through the bounds-checked buffer view `rs` (cand_rsrc: one
+// base offset, the slots as immediate offsets, three 16-byte loads per four
+// candidates from the (x,y,z) copy, else one 12-byte load each from pos4;
+// slots past the row end are loaded but never accepted), writes every
+// candidate into the next queue slot and
 // advances its count only on acceptance.  The caller keeps
 // cnt <= kQueue - kUnroll on entry.  The exact test (Within32) runs its FP64
 // branch only when a candidate of some lane falls in the +-2e-6 band; with
@@ -227,13 +233,30 @@ __device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buff
                                              float r2hi, double r2d, bool super, float rl2f,
                                              int* q, int& cnt) {
   float3 c[U];
-  int off = (int)((unsigned)j0 * 16u);
-  // keep off opaque, so that the slots become immediate load offsets
-  // instead of one (j0 + u) * 16 per slot (an empty register constraint:
-  // no instruction is emitted)
-  asm("" : "+v"(off));
+  if (S.pos3 != nullptr && U % 4 == 0) {
+    // (x,y,z) copy: four candidates are 48 contiguous bytes, three loads
+    int off = (int)((unsigned)j0 * 12u);
+    // keep off opaque, so that the slots become immediate load offsets
+    // (an empty register constraint: no instruction is emitted)
+    asm("" : "+v"(off));
 #pragma unroll
-  for (int u = 0; u < U; ++u) c[u] = ld_xyz(rs, off + 16 * u);
+    for (int g = 0; g < U / 4; ++g) {
+      float w[12];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 48 * g + 16 * k, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[4 * k + e] = __uint_as_float(v[e]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) c[4 * g + u] = make_float3(w[3 * u], w[3 * u + 1], w[3 * u + 2]);
+    }
+  } else {
+    int off = (int)((unsigned)j0 * 16u);
+    asm("" : "+v"(off));
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = ld_xyz(rs, off + 16 * u);
+  }
   const int rem = j1 - j0;
   bool ok[U], bnd[U];
   bool band = false;
@@ -346,7 +369,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
   int lpos = 0;
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
-    const __amdgpu_buffer_rsrc_t rs = pos4_rsrc(S);
+    const __amdgpu_buffer_rsrc_t rs = cand_rsrc(S);
     CT nr = CT(0);
     if (!BUILD && s == a.cache_src && a.cache_mode == 2) {
       // stored partial sum of a static source (exact round trip via double)
@@ -555,7 +578,7 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
   CT dv[3] = {CT(0), CT(0), CT(0)};
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
-    const __amdgpu_buffer_rsrc_t rs = pos4_rsrc(S);
+    const __amdgpu_buffer_rsrc_t rs = cand_rsrc(S);
     const bool do_m = (a.flags[s] & SPH_SRC_MOMENTUM) != 0 && a.do_mom;
     const bool do_x = (a.flags[s] & SPH_SRC_XSPH) != 0 && a.do_xsph;
     if (!do_m && !do_x) continue;

@@ -170,11 +170,18 @@ __global__ __launch_bounds__(kBlock) void momentum_list(Grid g, PassArgs<float> 
 // packing and launchers
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __restrict__ pos4,
-                                                      float4* __restrict__ vel4) {
+                                                      float4* __restrict__ vel4,
+                                                      float* __restrict__ pos3) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= a.n) return;
-  if (pos4 != nullptr) pos4[i] = make_float4(a.x[i], a.y[i], a.z[i], a.h[i]);
+  const float x = a.x[i], y = a.y[i], z = a.z[i];
+  if (pos4 != nullptr) pos4[i] = make_float4(x, y, z, a.h[i]);
   if (vel4 != nullptr) vel4[i] = make_float4(a.u[i], a.v[i], a.w[i], a.m[i]);
+  if (pos3 != nullptr) {
+    pos3[3 * (int64_t)i] = x;
+    pos3[3 * (int64_t)i + 1] = y;
+    pos3[3 * (int64_t)i + 2] = z;
+  }
 }
 
 // Compile-time dimension of the FP32-pair scan kernels (0: from the
@@ -282,7 +289,8 @@ int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<f
 
 using namespace sph;
 
-extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, void* vel4_dev) {
+extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, void* vel4_dev,
+                            void* pos3_dev) {
   SPH_TRY(check_view(a, "sph_pack", vel4_dev != nullptr, false, false));
   if (a->dtype != SPH_F32) {
     set_error("sph_pack: packed copies exist for FP32 storage only");
@@ -294,6 +302,6 @@ extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, voi
   }
   if (a->n == 0) return SPH_OK;
   hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, as_stream(stream),
-                     to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev);
+                     to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev);
   return launch_status("sph_pack");
 }

@@ -231,10 +231,11 @@ class WCSPHPlan:
 
     # ------------------------------------------------------ packed copies
     # FP32 storage: every array carries float4 copies (x,y,z,h), (u,v,w,m)
-    # and (rho,cs,p/rho^2,p) in its current slot order, read by the fast
-    # neighbour passes.  pos4/vel4 are re-packed after every binning of the
-    # array (and from X1 after the stage-2 binning); aux4 is written by the
-    # density pass that precedes every momentum pass.
+    # and (rho,cs,p/rho^2,p), and an (x,y,z) copy for the candidate screen,
+    # in its current slot order, read by the fast neighbour passes.
+    # pos4/vel4/pos3 are re-packed after every binning of the array (and
+    # from X1 for stage 2); aux4 is written by the density pass that
+    # precedes every momentum pass.
     def _packed(self, pa):
         if not self.use_packed:
             return None
@@ -243,7 +244,9 @@ class WCSPHPlan:
         if pk is None or pk["n"] != n:
             mk = lambda: torch.empty((max(n, 1), 4), dtype=torch.float32,  # noqa: E731
                                      device=pa.device)
-            pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(), "version": None}
+            pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(),
+                  "pos3": torch.empty(max(n, 1) * 3, dtype=torch.float32, device=pa.device),
+                  "version": None}
             self._pk[id(pa)] = pk
         return pk
 
@@ -258,7 +261,8 @@ class WCSPHPlan:
         v = pa.view()
         for f, t in (fields or {}).items():
             setattr(v, f, ptr(t))
-        _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr())
+        _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr(),
+                  pk["pos3"].data_ptr())
         pk["version"] = ver
 
     def _pack_all(self):
@@ -272,7 +276,8 @@ class WCSPHPlan:
         if self.use_packed:
             for pa in self.particles.arrays:
                 pk = self._pk[id(pa)]
-                ov[id(pa)] = {"pos4": pk["pos4"], "vel4": pk["vel4"], "aux4": pk["aux4"]}
+                ov[id(pa)] = {"pos4": pk["pos4"], "vel4": pk["vel4"], "aux4": pk["aux4"],
+                              "pos3": pk["pos3"]}
         if s1 is not None:
             ov.setdefault(id(self.fluid), {}).update(s1)
         return ov
