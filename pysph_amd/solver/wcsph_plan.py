@@ -46,6 +46,9 @@ from ..sph.funcs import (GravityForce, MomentumEquation, PositionStepping, SPHRh
 
 
 class WCSPHPlan:
+    # moving-array size from which the density passes run concurrently
+    CONCURRENT_MIN = 1 << 18
+
     def __init__(self, particles, rho_calcs, tait, mom, grav, xsph, dim, radius_scale):
         self.particles = particles
         self.cm = particles.cell_manager
@@ -109,8 +112,11 @@ class WCSPHPlan:
         self._caches = {}
         # concurrent_density: the density passes of the arrays other than
         # the moving one run on a second stream (default priority), beside
-        # the moving array's pass; the stage kernel waits for both
-        self.concurrent_density = True
+        # the moving array's pass; the stage kernel waits for both.  None
+        # (auto): on when the moving array has at least CONCURRENT_MIN
+        # particles; below that the launches are short and the cross-stream
+        # waits cost more than the overlap gains
+        self.concurrent_density = None
         self._side = None
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
@@ -348,7 +354,10 @@ class WCSPHPlan:
         prm = self.prm
         main = torch.cuda.current_stream()
         side = None
-        if self.concurrent_density and len(self.rho_calcs) > 1:
+        conc = self.concurrent_density
+        if conc is None:
+            conc = self.fluid.get_number_of_particles() >= self.CONCURRENT_MIN
+        if conc and len(self.rho_calcs) > 1:
             if self._side is None:
                 self._side = torch.cuda.Stream(device=self.fluid.device)
             side = self._side
