@@ -53,6 +53,9 @@ class ParticleArray:
         d["_kinds"] = {}
         d["default_values"] = {"tag": default_particle_tag, "group": 0, "local": 1}
         d["_order"] = None  # None: identity
+        # properties a re-binning leaves in place (set by an integrator whose
+        # fused path never reads them; their values are then meaningless)
+        d["_unpermuted"] = set()
         d["_n"] = 0
         # binning state written by CellManager
         d["_cell"] = None
@@ -279,8 +282,9 @@ class ParticleArray:
         if self._order is None:
             self.__dict__["_order"] = torch.arange(n, dtype=torch.int64, device=self.device)
         extra = list(extra)
-        names = list(self._props) + ["__order__"] + ["__extra__"] * len(extra)
-        srcs = [self._props[k] for k in self._props] + [self._order] + extra
+        moved = [k for k in self._props if k not in self._unpermuted]
+        names = moved + ["__order__"] + ["__extra__"] * len(extra)
+        srcs = [self._props[k] for k in moved] + [self._order] + extra
         dsts = [torch.empty_like(s) for s in srcs]
         esz = (ctypes_int32 * len(srcs))(*[s.element_size() for s in srcs])
         _abi.call("sph_permute", stream, n, perm.data_ptr(), len(srcs),

@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void bin_rank(int n, const int* __restrict_
 // ---------------------------------------------------------------------------
 // permutation gather
 // ---------------------------------------------------------------------------
-constexpr int kPermFields = 16;
+constexpr int kPermFields = 24;
 struct PermArgs {
   const void* src[kPermFields];
   void* dst[kPermFields];
@@ -283,16 +283,29 @@ struct PermArgs {
   int nf;
 };
 
+// Every field's load is issued before the first store (no output aliases
+// an input: the entry point checks), so a thread has up to kPermFields
+// gathers in flight instead of one.
 __global__ __launch_bounds__(kBlock) void permute_kernel(int n, const int* __restrict__ perm,
                                                          PermArgs a) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
   const int j = perm[p];
-  for (int f = 0; f < a.nf; ++f) {
-    if (a.esz[f] == 4)
-      ((uint32_t*)a.dst[f])[p] = ((const uint32_t*)a.src[f])[j];
-    else
-      ((uint64_t*)a.dst[f])[p] = ((const uint64_t*)a.src[f])[j];
+  uint64_t v[kPermFields];
+#pragma unroll
+  for (int f = 0; f < kPermFields; ++f) {
+    if (f < a.nf)
+      v[f] = a.esz[f] == 4 ? (uint64_t)((const uint32_t*)a.src[f])[j]
+                           : ((const uint64_t*)a.src[f])[j];
+  }
+#pragma unroll
+  for (int f = 0; f < kPermFields; ++f) {
+    if (f < a.nf) {
+      if (a.esz[f] == 4)
+        ((uint32_t*)a.dst[f])[p] = (uint32_t)v[f];
+      else
+        ((uint64_t*)a.dst[f])[p] = v[f];
+    }
   }
 }
 
@@ -549,6 +562,12 @@ int32_t sph_permute(void* stream, int64_t n, const int32_t* perm_dev, int32_t nf
     return SPH_EINVAL;
   }
   hipStream_t st = as_stream(stream);
+  for (int f = 0; f < nfields; ++f)
+    for (int g = 0; g < nfields; ++g)
+      if (n > 0 && dst_dev_host[f] != nullptr && dst_dev_host[f] == src_dev_host[g]) {
+        set_error("sph_permute: output %d aliases input %d", f, g);
+        return SPH_EINVAL;
+      }
   for (int f0 = 0; f0 < nfields; f0 += kPermFields) {
     PermArgs pa;
     pa.nf = nfields - f0 < kPermFields ? nfields - f0 : kPermFields;

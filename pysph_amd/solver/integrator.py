@@ -79,6 +79,18 @@ class Integrator:
             self.plan = WCSPHPlan.match(self.particles, calcs)
             if self.plan is not None:
                 logger.info("integrator: fused WCSPH plan on array %s", self.plan.fluid.name)
+        # the fused plan keeps its own stage buffers: the _p_0 / _k arrays
+        # stay allocated (as the reference creates them) but are not carried
+        # through re-binnings, and hold no meaningful values while it runs
+        scratch = {}
+        for calc in calcs:
+            pa = self.arrays[calc.dnum]
+            names = scratch.setdefault(id(pa), set())
+            names.update(calc.initial_props)
+            for ks in self.k_props[calc.id].values():
+                names.update(ks)
+        for pa in self.arrays:
+            pa.__dict__["_unpermuted"] = set(scratch.get(id(pa), ())) if self.plan else set()
         self.setup_done = True
 
     # ------------------------------------------------------------ generic path
