@@ -154,7 +154,9 @@ int32_t sph_bin(void *stream, const sph_view *a, const sph_grid *g,
 
 /* Gather nfields arrays through perm: dst[f][p] = src[f][perm[p]].
  * elem_bytes[f] is 4 or 8.  Host arrays of device pointers; no output may
- * be one of the inputs. */
+ * be one of the inputs.  Runs of consecutive sources are copied 16 bytes
+ * at a time when perm and every output are 16-byte aligned (as device
+ * allocations are); otherwise element by element. */
 int32_t sph_permute(void *stream, int64_t n, const int32_t *perm_dev,
                     int32_t nfields, const void *const *src_dev_host,
                     void *const *dst_dev_host, const int32_t *elem_bytes_host);

@@ -283,29 +283,92 @@ struct PermArgs {
   int nf;
 };
 
-// Every field's load is issued before the first store (no output aliases
-// an input: the entry point checks), so a thread has up to kPermFields
-// gathers in flight instead of one.
+// One thread moves four consecutive slots p..p+3 of every field.  Slots
+// whose sources are consecutive too (a run: the common case, since few
+// particles change cell between two binnings) are copied with one 16-byte
+// load per 4-byte field (two per 8-byte field); other groups gather the
+// four elements one by one.  Loads go through bounds-checked buffer
+// resources (dword-aligned 16-byte reads are allowed there); the stores are
+// 16-byte aligned because p is a multiple of 4.  No output aliases an input
+// (the entry point checks).
+constexpr int kPermPer = 4;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t perm_rsrc(const void* base, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)(uint32_t)bytes, 0x00020000);
+}
+
 __global__ __launch_bounds__(kBlock) void permute_kernel(int n, const int* __restrict__ perm,
                                                          PermArgs a) {
+  const int p = (blockIdx.x * kBlock + threadIdx.x) * kPermPer;
+  if (p >= n) return;
+  const bool full = p + kPermPer <= n;
+  int j[kPermPer];
+  if (full) {
+    const int4 jj = *(const int4*)(perm + p);
+    j[0] = jj.x; j[1] = jj.y; j[2] = jj.z; j[3] = jj.w;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kPermPer; ++k) j[k] = p + k < n ? perm[p + k] : 0;
+  }
+  const bool run = full && j[1] == j[0] + 1 && j[2] == j[0] + 2 && j[3] == j[0] + 3;
+#pragma unroll 4
+  for (int f = 0; f < a.nf; ++f) {
+    const int esz = a.esz[f];
+    const __amdgpu_buffer_rsrc_t rs = perm_rsrc(a.src[f], (int64_t)n * esz);
+    if (esz == 4) {
+      uint32_t* __restrict__ d = (uint32_t*)a.dst[f] + p;
+      if (run) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, j[0] * 4, 0, 0);
+        *(uint4*)d = make_uint4(v[0], v[1], v[2], v[3]);
+      } else {
+        uint32_t v[kPermPer];
+#pragma unroll
+        for (int k = 0; k < kPermPer; ++k) v[k] = __builtin_amdgcn_raw_buffer_load_b32(rs, j[k] * 4, 0, 0);
+        if (full) {
+          *(uint4*)d = make_uint4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+          for (int k = 0; k < kPermPer; ++k)
+            if (p + k < n) d[k] = v[k];
+        }
+      }
+    } else {
+      uint64_t* __restrict__ d = (uint64_t*)a.dst[f] + p;
+      if (run) {
+        const auto v0 = __builtin_amdgcn_raw_buffer_load_b128(rs, j[0] * 8, 0, 0);
+        const auto v1 = __builtin_amdgcn_raw_buffer_load_b128(rs, j[0] * 8 + 16, 0, 0);
+        *(uint4*)d = make_uint4(v0[0], v0[1], v0[2], v0[3]);
+        *((uint4*)d + 1) = make_uint4(v1[0], v1[1], v1[2], v1[3]);
+      } else {
+        uint64_t v[kPermPer];
+#pragma unroll
+        for (int k = 0; k < kPermPer; ++k) {
+          const auto w = __builtin_amdgcn_raw_buffer_load_b64(rs, j[k] * 8, 0, 0);
+          v[k] = (uint64_t)w[0] | ((uint64_t)w[1] << 32);
+        }
+#pragma unroll
+        for (int k = 0; k < kPermPer; ++k)
+          if (p + k < n) d[k] = v[k];
+      }
+    }
+  }
+}
+
+// Arrays past kPermRunMax elements (byte offsets beyond 32 bits), and
+// outputs or permutations that are not 16-byte aligned, take the
+// element-wise gather.
+constexpr int64_t kPermRunMax = (int64_t)1 << 28;
+
+__global__ __launch_bounds__(kBlock) void permute_kernel_wide(int n, const int* __restrict__ perm,
+                                                              PermArgs a) {
   const int p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= n) return;
-  const int j = perm[p];
-  uint64_t v[kPermFields];
-#pragma unroll
-  for (int f = 0; f < kPermFields; ++f) {
-    if (f < a.nf)
-      v[f] = a.esz[f] == 4 ? (uint64_t)((const uint32_t*)a.src[f])[j]
-                           : ((const uint64_t*)a.src[f])[j];
-  }
-#pragma unroll
-  for (int f = 0; f < kPermFields; ++f) {
-    if (f < a.nf) {
-      if (a.esz[f] == 4)
-        ((uint32_t*)a.dst[f])[p] = (uint32_t)v[f];
-      else
-        ((uint64_t*)a.dst[f])[p] = v[f];
-    }
+  const int64_t j = perm[p];
+  for (int f = 0; f < a.nf; ++f) {
+    if (a.esz[f] == 4)
+      ((uint32_t*)a.dst[f])[p] = ((const uint32_t*)a.src[f])[j];
+    else
+      ((uint64_t*)a.dst[f])[p] = ((const uint64_t*)a.src[f])[j];
   }
 }
 
@@ -583,9 +646,16 @@ int32_t sph_permute(void* stream, int64_t n, const int32_t* perm_dev, int32_t nf
       pa.dst[k] = dst_dev_host[f];
       pa.esz[k] = elem_bytes_host[f];
     }
-    if (n > 0)
-      hipLaunchKernelGGL(permute_kernel, dim3(nblocks_for(n)), dim3(kBlock), 0, st, (int)n,
+    if (n == 0) continue;
+    // the run kernel reads perm and writes every output 16 bytes at a time
+    bool aligned = ((uintptr_t)perm_dev & 15) == 0;
+    for (int k = 0; k < pa.nf; ++k) aligned = aligned && ((uintptr_t)pa.dst[k] & 15) == 0;
+    if (n > kPermRunMax || !aligned)
+      hipLaunchKernelGGL(permute_kernel_wide, dim3(nblocks_for(n)), dim3(kBlock), 0, st, (int)n,
                          perm_dev, pa);
+    else
+      hipLaunchKernelGGL(permute_kernel, dim3(nblocks_for(n, kBlock * kPermPer)), dim3(kBlock), 0,
+                         st, (int)n, perm_dev, pa);
   }
   return launch_status("sph_permute");
 }
