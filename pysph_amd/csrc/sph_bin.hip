@@ -200,6 +200,32 @@ __device__ __forceinline__ int grid_coord(double x, double cs, int lo, int dim) 
   return (int)c;
 }
 
+// Runs of one cell among consecutive lanes of a wave.  Particles arrive in
+// the slot order of the previous binning, i.e. mostly cell-sorted, so a
+// run's first lane can do the run's atomic for all of it.  Valid lanes are
+// a prefix of the wave (only the last wave of a launch has invalid lanes).
+struct WaveRun {
+  int head;      // lane of this lane's run head
+  int len;       // run length (heads only)
+  bool is_head;
+};
+
+__device__ __forceinline__ WaveRun wave_run(int key, bool valid) {
+  const int lane = threadIdx.x & 63;
+  const int prev = __shfl_up(key, 1, 64);
+  WaveRun r;
+  r.is_head = valid && (lane == 0 || prev != key);
+  const unsigned long long heads = __ballot(r.is_head);
+  const unsigned long long vmask = __ballot(valid);
+  const unsigned long long upto = lane == 63 ? ~0ull : ((2ull << lane) - 1);
+  const unsigned long long mine = heads & upto;
+  r.head = mine ? 63 - __clzll((long long)mine) : lane;
+  const unsigned long long after = heads & ~upto;
+  const int end = after ? __ffsll((long long)after) - 1 : (vmask ? 64 - __clzll((long long)vmask) : 0);
+  r.len = end - lane;
+  return r;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
                                                     int* __restrict__ cell_tmp,
@@ -207,14 +233,18 @@ __global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
                                                     const int* __restrict__ prev_cell,
                                                     int* __restrict__ jumps) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  const int cx = grid_coord((double)a.x[i], g.cs, g.lo[0], g.dims[0]);
-  const int cy = grid_coord((double)a.y[i], g.cs, g.lo[1], g.dims[1]);
-  const int cz = grid_coord((double)a.z[i], g.cs, g.lo[2], g.dims[2]);
-  const int c = (cz * g.dims[1] + cy) * g.dims[0] + cx;
-  cell_tmp[i] = c;
-  atomicAdd(&counts[c], 1);
-  if (prev_cell != nullptr) {
+  const bool valid = i < a.n;
+  int c = -1, cx = 0, cy = 0, cz = 0;
+  if (valid) {
+    cx = grid_coord((double)a.x[i], g.cs, g.lo[0], g.dims[0]);
+    cy = grid_coord((double)a.y[i], g.cs, g.lo[1], g.dims[1]);
+    cz = grid_coord((double)a.z[i], g.cs, g.lo[2], g.dims[2]);
+    c = (cz * g.dims[1] + cy) * g.dims[0] + cx;
+    cell_tmp[i] = c;
+  }
+  const WaveRun r = wave_run(c, valid);
+  if (r.is_head) atomicAdd(&counts[c], r.len);
+  if (valid && prev_cell != nullptr) {
     const int pc = prev_cell[i];
     const int px = pc % g.dims[0], t = pc / g.dims[0];
     const int py = t % g.dims[1], pz = t / g.dims[1];
@@ -233,14 +263,18 @@ __global__ __launch_bounds__(kBlock) void batch_bin_count(int n, const float* __
                                                           int* __restrict__ cell_tmp,
                                                           int* __restrict__ counts) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const BatchBinGrid g = grids[scene[i]];
-  const int cx = grid_coord((double)x[i], g.cs, g.lo[0], g.dims[0]);
-  const int cy = grid_coord((double)y[i], g.cs, g.lo[1], g.dims[1]);
-  const int cz = grid_coord((double)z[i], g.cs, g.lo[2], g.dims[2]);
-  const int c = g.cell_base + (cz * g.dims[1] + cy) * g.dims[0] + cx;
-  cell_tmp[i] = c;
-  atomicAdd(&counts[c], 1);
+  const bool valid = i < n;
+  int c = -1;
+  if (valid) {
+    const BatchBinGrid g = grids[scene[i]];
+    const int cx = grid_coord((double)x[i], g.cs, g.lo[0], g.dims[0]);
+    const int cy = grid_coord((double)y[i], g.cs, g.lo[1], g.dims[1]);
+    const int cz = grid_coord((double)z[i], g.cs, g.lo[2], g.dims[2]);
+    c = g.cell_base + (cz * g.dims[1] + cy) * g.dims[0] + cx;
+    cell_tmp[i] = c;
+  }
+  const WaveRun r = wave_run(c, valid);
+  if (r.is_head) atomicAdd(&counts[c], r.len);
 }
 
 __global__ __launch_bounds__(kBlock) void bin_scatter(int n, const int* __restrict__ cell_tmp,
@@ -248,10 +282,15 @@ __global__ __launch_bounds__(kBlock) void bin_scatter(int n, const int* __restri
                                                       int* __restrict__ cursor,
                                                       int* __restrict__ bucket) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const int c = cell_tmp[i];
-  const int pos = cstart[c] + atomicAdd(&cursor[c], 1);
-  bucket[pos] = i;
+  const bool valid = i < n;
+  const int c = valid ? cell_tmp[i] : -1;
+  // one atomic per run of a cell; the run's members take consecutive
+  // positions in lane order
+  const WaveRun r = wave_run(c, valid);
+  int base = 0;
+  if (r.is_head) base = atomicAdd(&cursor[c], r.len);
+  base = __shfl(base, r.head, 64);
+  if (valid) bucket[cstart[c] + base + ((threadIdx.x & 63) - r.head)] = i;
 }
 
 // The atomic scatter leaves each cell's members in arbitrary order; rank
