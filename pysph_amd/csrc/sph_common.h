@@ -198,6 +198,8 @@ struct View {
   const float4* vel4;  // packed (u,v,w,m)
   float4* aux4;        // packed (rho,cs,p/rho^2,p)
   const float* pos3;   // packed (x,y,z), 3 floats per particle
+  const float4* pm4;   // packed (x,y,z,m)
+  const int* hrange;   // smallest and largest h (FP32 bits)
 };
 
 template <typename T>
@@ -213,6 +215,8 @@ inline View<T> to_view(const sph_view& a) {
   v.vel4 = (const float4*)a.vel4;
   v.aux4 = (float4*)a.aux4;
   v.pos3 = (const float*)a.pos3;
+  v.pm4 = (const float4*)a.pm4;
+  v.hrange = (const int*)a.hrange;
   return v;
 }
 

@@ -298,14 +298,21 @@ __device__ __forceinline__ bool list_valid(const float* disp, float skin, double
 // ---------------------------------------------------------------------------
 // density
 // ---------------------------------------------------------------------------
+// m W(r_a - r_b, (h_a + h_b) / 2); ps carries the source position.
 template <typename CT>
-__device__ __forceinline__ CT w_term(const float4& pd, const float4& ps, CT hd, CT m, int dim) {
+__device__ __forceinline__ CT w_term_hb(const float4& pd, const float4& ps, CT hd, CT hb, CT m,
+                                        int dim) {
   CT rx, ry, rz;
   const CT d2 = pair_d2<CT>(pd, ps, rx, ry, rz);
-  const CT hab = CT(0.5) * (hd + (CT)ps.w);
+  const CT hab = CT(0.5) * (hd + hb);
   const CT inv = frcp<CT>(hab);
   const CT qq = fsqrt<CT>(d2) * inv;
   return m * (w_sel<CT>(qq) * h_dim_inv<CT>(inv, dim));
+}
+
+template <typename CT>
+__device__ __forceinline__ CT w_term(const float4& pd, const float4& ps, CT hd, CT m, int dim) {
+  return w_term_hb<CT>(pd, ps, hd, (CT)ps.w, m, dim);
 }
 
 template <bool TAIT>
@@ -378,14 +385,30 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
       continue;
     }
     int cnt = 0, cnt_s = 0;
+    // a source of uniform h with an (x,y,z,m) copy: one 16-byte record per
+    // pair, h_b from the source's h range (the same value, the same bits)
+    const bool uni_h = exact_list && S.pm4 != nullptr && S.hrange != nullptr &&
+                       S.hrange[0] == S.hrange[1];
+    const CT h_src = uni_h ? (CT)__int_as_float(S.hrange[0]) : CT(0);
     auto drain = [&]() {
-      for (int k = 0; __any(k < cnt); ++k) {
-        if (k < cnt) {
-          const int j = q[k * 64];
-          const float4 ps = ld4(S.pos4, j);
-          if (exact_list || accept_exact(ps, pd, r2f, r2d))
-            nr += w_term<CT>(pd, ps, hd, (CT)ld4(S.vel4, j).w, dim);
-          if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
+      if (uni_h) {
+        for (int k = 0; __any(k < cnt); ++k) {
+          if (k < cnt) {
+            const int j = q[k * 64];
+            const float4 pm = ld4(S.pm4, j);
+            nr += w_term_hb<CT>(pd, pm, hd, h_src, (CT)pm.w, dim);
+            if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
+          }
+        }
+      } else {
+        for (int k = 0; __any(k < cnt); ++k) {
+          if (k < cnt) {
+            const int j = q[k * 64];
+            const float4 ps = ld4(S.pos4, j);
+            if (exact_list || accept_exact(ps, pd, r2f, r2d))
+              nr += w_term<CT>(pd, ps, hd, (CT)ld4(S.vel4, j).w, dim);
+            if (write_list && lpos + k < a.nl_max) a.nl_idx[(int64_t)(lpos + k) * ndst + i] = j;
+          }
         }
       }
       lpos += cnt;

@@ -171,17 +171,56 @@ __global__ __launch_bounds__(kBlock) void momentum_list(Grid g, PassArgs<float> 
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __restrict__ pos4,
                                                       float4* __restrict__ vel4,
-                                                      float* __restrict__ pos3) {
+                                                      float* __restrict__ pos3,
+                                                      float4* __restrict__ pm4,
+                                                      int* __restrict__ hrange) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.n) return;
-  const float x = a.x[i], y = a.y[i], z = a.z[i];
-  if (pos4 != nullptr) pos4[i] = make_float4(x, y, z, a.h[i]);
-  if (vel4 != nullptr) vel4[i] = make_float4(a.u[i], a.v[i], a.w[i], a.m[i]);
-  if (pos3 != nullptr) {
-    pos3[3 * (int64_t)i] = x;
-    pos3[3 * (int64_t)i + 1] = y;
-    pos3[3 * (int64_t)i + 2] = z;
+  const bool valid = i < a.n;
+  if (valid) {
+    const float x = a.x[i], y = a.y[i], z = a.z[i];
+    if (pos4 != nullptr) pos4[i] = make_float4(x, y, z, a.h[i]);
+    if (vel4 != nullptr) vel4[i] = make_float4(a.u[i], a.v[i], a.w[i], a.m[i]);
+    if (pm4 != nullptr) pm4[i] = make_float4(x, y, z, a.m[i]);
+    if (pos3 != nullptr) {
+      pos3[3 * (int64_t)i] = x;
+      pos3[3 * (int64_t)i + 1] = y;
+      pos3[3 * (int64_t)i + 2] = z;
+    }
   }
+  if (hrange != nullptr) {
+    // h >= 0 orders like its bit pattern.  Block minimum and maximum, then
+    // an atomic only where the block improves the stored value (read past
+    // L1): with uniform h nearly every block skips both atomics.
+    __shared__ int red[2][kBlock / 64];
+    int lo = valid ? __float_as_int(a.h[i]) : 0x7f7fffff;
+    int hi = valid ? lo : 0;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      lo = min(lo, __shfl_xor(lo, off, 64));
+      hi = max(hi, __shfl_xor(hi, off, 64));
+    }
+    if ((threadIdx.x & 63) == 0) {
+      red[0][threadIdx.x >> 6] = lo;
+      red[1][threadIdx.x >> 6] = hi;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int w = 1; w < kBlock / 64; ++w) {
+        lo = min(lo, red[0][w]);
+        hi = max(hi, red[1][w]);
+      }
+      if (lo < __hip_atomic_load(&hrange[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(&hrange[0], lo);
+      if (hi > __hip_atomic_load(&hrange[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMax(&hrange[1], hi);
+    }
+  }
+}
+
+__global__ void hrange_init(int* __restrict__ hrange) {
+  hrange[0] = 0x7f7fffff;
+  hrange[1] = 0;
 }
 
 // Compile-time dimension of the FP32-pair scan kernels (0: from the
@@ -290,18 +329,21 @@ int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<f
 using namespace sph;
 
 extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, void* vel4_dev,
-                            void* pos3_dev) {
+                            void* pos3_dev, void* pm4_dev, int32_t* hrange_dev) {
   SPH_TRY(check_view(a, "sph_pack", vel4_dev != nullptr, false, false));
   if (a->dtype != SPH_F32) {
     set_error("sph_pack: packed copies exist for FP32 storage only");
     return SPH_EUNSUPPORTED;
   }
-  if (vel4_dev != nullptr && a->n > 0 && !a->m) {
-    set_error("sph_pack: m is required for vel4");
+  if ((vel4_dev != nullptr || pm4_dev != nullptr) && a->n > 0 && !a->m) {
+    set_error("sph_pack: m is required for vel4 and pm4");
     return SPH_EINVAL;
   }
   if (a->n == 0) return SPH_OK;
-  hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, as_stream(stream),
-                     to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev);
+  hipStream_t st = as_stream(stream);
+  if (hrange_dev != nullptr) hipLaunchKernelGGL(hrange_init, dim3(1), dim3(1), 0, st, hrange_dev);
+  hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, st,
+                     to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev,
+                     (float4*)pm4_dev, (int*)hrange_dev);
   return launch_status("sph_pack");
 }

@@ -236,12 +236,12 @@ class WCSPHPlan:
         return self._bufs
 
     # ------------------------------------------------------ packed copies
-    # FP32 storage: every array carries float4 copies (x,y,z,h), (u,v,w,m)
-    # and (rho,cs,p/rho^2,p), and an (x,y,z) copy for the candidate screen,
-    # in its current slot order, read by the fast neighbour passes.
-    # pos4/vel4/pos3 are re-packed after every binning of the array (and
-    # from X1 for stage 2); aux4 is written by the density pass that
-    # precedes every momentum pass.
+    # FP32 storage: every array carries float4 copies (x,y,z,h), (u,v,w,m),
+    # (rho,cs,p/rho^2,p) and (x,y,z,m), an (x,y,z) copy for the candidate
+    # screen and its h range, in its current slot order, read by the fast
+    # neighbour passes.  All but aux4 are re-packed after every binning of
+    # the array (and from X1 for stage 2); aux4 is written by the density
+    # pass that precedes every momentum pass.
     def _packed(self, pa):
         if not self.use_packed:
             return None
@@ -250,8 +250,9 @@ class WCSPHPlan:
         if pk is None or pk["n"] != n:
             mk = lambda: torch.empty((max(n, 1), 4), dtype=torch.float32,  # noqa: E731
                                      device=pa.device)
-            pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(),
+            pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(), "pm4": mk(),
                   "pos3": torch.empty(max(n, 1) * 3, dtype=torch.float32, device=pa.device),
+                  "hrange": torch.empty(2, dtype=torch.int32, device=pa.device),
                   "version": None}
             self._pk[id(pa)] = pk
         return pk
@@ -268,7 +269,7 @@ class WCSPHPlan:
         for f, t in (fields or {}).items():
             setattr(v, f, ptr(t))
         _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr(),
-                  pk["pos3"].data_ptr())
+                  pk["pos3"].data_ptr(), pk["pm4"].data_ptr(), pk["hrange"].data_ptr())
         pk["version"] = ver
 
     def _pack_all(self):
@@ -283,7 +284,7 @@ class WCSPHPlan:
             for pa in self.particles.arrays:
                 pk = self._pk[id(pa)]
                 ov[id(pa)] = {"pos4": pk["pos4"], "vel4": pk["vel4"], "aux4": pk["aux4"],
-                              "pos3": pk["pos3"]}
+                              "pos3": pk["pos3"], "pm4": pk["pm4"], "hrange": pk["hrange"]}
         if s1 is not None:
             ov.setdefault(id(self.fluid), {}).update(s1)
         return ov
