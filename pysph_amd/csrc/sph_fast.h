@@ -5,15 +5,19 @@
 // Packed FP32 fast path of the neighbour passes.
 //
 // Inputs are float4 copies per particle: pos4 = (x,y,z,h), vel4 = (u,v,w,m),
-// aux4 = (rho,cs,p/rho^2,p): the FP32 pair arithmetic reads the first three,
-// one 12-byte load.  One lane per destination particle, the
-// particles sorted by cell so that the lanes of a wave share stencil rows.
+// aux4 = (rho,cs,p/rho^2,p) (the FP32 pair arithmetic reads the first three,
+// one 12-byte load), optionally pm4 = (x,y,z,m) with the array's h range
+// (a density source of uniform h: one load per pair), and the (x,y,z) copy
+// pos3 that the candidate screen reads.  One lane per destination particle,
+// the particles sorted by cell so that the lanes of a wave share stencil
+// rows.
 //
-// Cell scans: a lane screens the candidates of its stencil rows (one 16-byte
-// load each, four in flight) and queues the accepted source indices in LDS;
-// when a queue is full, and after each source array, the wave drains all
-// queues in one wave-uniform loop, so the pair arithmetic runs with nearly
-// all lanes active instead of the ~15% that accept any one candidate.
+// Cell scans: a lane screens the candidates of its stencil rows (four per
+// step, three 16-byte loads from pos3) and queues the accepted source
+// indices in LDS; when a queue is full, and after each source array, the
+// wave drains all queues in one wave-uniform loop, so the pair arithmetic
+// runs with ~70% of lanes active instead of the ~40% that accept any one
+// candidate.
 //
 // Neighbour lists: a density scan can record the accepted sources of every
 // destination.  Without skin the list holds exactly the neighbours and the

@@ -58,6 +58,31 @@ def test_sphrho_two_sources(precision):
     assert max_rel(got, ref) < TOL[precision]
 
 
+@pytest.mark.parametrize("case", ["varied_fluid_h", "uniform_but_different_h"])
+def test_sphrho_sources_of_uniform_and_varied_h(case):
+    """The FP32 density pass reads one (x,y,z,m) record per pair from a
+    source whose h is uniform (h taken from its h range) and the pos4 / vel4
+    pair otherwise; both against the oracle, with fluid and boundary h
+    different from each other."""
+    f, b, info = _dam2d("single", seed=3)
+    rng = np.random.Generator(np.random.PCG64(9))
+    hf = f.get("h")
+    if case == "varied_fluid_h":
+        f.set(h=(hf * (0.9 + 0.2 * rng.random(len(hf)))).astype(np.float32))
+    else:
+        f.set(h=(hf * 1.1).astype(np.float32))
+    b.set(h=(b.get("h") * 0.95).astype(np.float32))
+    particles = base.Particles([f, b])
+    rho_f = sph.SPHRho.withargs()
+    for dst, srcs in ((f, [f, b]), (b, [f, b])):
+        calc = sph.SPHCalc(particles, srcs, dst, base.CubicSplineKernel(dim=2),
+                           [rho_f.get_func(s, dst) for s in srcs], updates=["rho"])
+        calc.sph("rho")
+        ref = O.sphrho(2, to_oracle(dst), [to_oracle(s) for s in srcs],
+                       cell_size=particles.cell_manager.cell_size)
+        assert max_rel(dst.get("rho"), ref) < TOL["single"]
+
+
 @pytest.mark.parametrize("precision", ["single", "double"])
 def test_tait_closed_form(precision):
     """p = B((rho/ro)^7 - 1), cs = co (rho/ro)^3 against an independent
