@@ -76,10 +76,11 @@ typedef struct sph_view {
      * passes reads four candidates in three 16-byte loads. */
     const void *pos3;
     /* Optional packed (x,y,z,m) FP32 copy (float4 per particle) and the
-     * array's smallest and largest h as FP32 bit patterns (2 int32 on the
-     * device), both from sph_pack: when a density source carries both and
-     * its h is uniform, the density pass reads one 16-byte record per pair
-     * (same arithmetic, same bits). */
+     * array's h-uniformity pair (2 int32 on the device: FP32 bit patterns,
+     * equal exactly when every particle has the same h, which is then
+     * hrange[0]), both from sph_pack: when a density source carries both
+     * and its h is uniform, the density pass reads one 16-byte record per
+     * pair (same arithmetic, same bits). */
     const void *pm4;
     const int32_t *hrange;
 } sph_view;
@@ -181,8 +182,9 @@ int32_t sph_periodic_wrap(void *stream, const sph_grid *g, int32_t dtype, int64_
 
 /* Write the packed copies pos4[i] = (x,y,z,h), vel4[i] = (u,v,w,m),
  * pm4[i] = (x,y,z,m) (float4 each) and pos3[i] = (x,y,z) (3 floats) of an
- * FP32 array, and hrange[0..1] = the bit patterns of its smallest and
- * largest h (h >= 0); any output may be NULL. */
+ * FP32 array, and its h-uniformity pair hrange[0..1] (the bit pattern of
+ * h[0] twice when every h equals it; otherwise hrange[1] holds another h);
+ * any output may be NULL. */
 int32_t sph_pack(void *stream, const sph_view *a, void *pos4_dev, void *vel4_dev,
                  void *pos3_dev, void *pm4_dev, int32_t *hrange_dev);
 

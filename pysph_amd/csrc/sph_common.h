@@ -199,7 +199,7 @@ struct View {
   float4* aux4;        // packed (rho,cs,p/rho^2,p)
   const float* pos3;   // packed (x,y,z), 3 floats per particle
   const float4* pm4;   // packed (x,y,z,m)
-  const int* hrange;   // smallest and largest h (FP32 bits)
+  const int* hrange;   // h-uniformity pair (FP32 bits; equal: uniform h)
 };
 
 template <typename T>

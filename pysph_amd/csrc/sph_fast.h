@@ -390,7 +390,8 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
     }
     int cnt = 0, cnt_s = 0;
     // a source of uniform h with an (x,y,z,m) copy: one 16-byte record per
-    // pair, h_b from the source's h range (the same value, the same bits)
+    // pair, h_b from the source's h-uniformity pair (the same value, the
+    // same bits)
     const bool uni_h = exact_list && S.pm4 != nullptr && S.hrange != nullptr &&
                        S.hrange[0] == S.hrange[1];
     const CT h_src = uni_h ? (CT)__int_as_float(S.hrange[0]) : CT(0);

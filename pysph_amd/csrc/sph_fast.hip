@@ -187,40 +187,18 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __r
       pos3[3 * (int64_t)i + 2] = z;
     }
   }
-  if (hrange != nullptr) {
-    // h >= 0 orders like its bit pattern.  Block minimum and maximum, then
-    // an atomic only where the block improves the stored value (read past
-    // L1): with uniform h nearly every block skips both atomics.
-    __shared__ int red[2][kBlock / 64];
-    int lo = valid ? __float_as_int(a.h[i]) : 0x7f7fffff;
-    int hi = valid ? lo : 0;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      lo = min(lo, __shfl_xor(lo, off, 64));
-      hi = max(hi, __shfl_xor(hi, off, 64));
-    }
-    if ((threadIdx.x & 63) == 0) {
-      red[0][threadIdx.x >> 6] = lo;
-      red[1][threadIdx.x >> 6] = hi;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-      for (int w = 1; w < kBlock / 64; ++w) {
-        lo = min(lo, red[0][w]);
-        hi = max(hi, red[1][w]);
-      }
-      if (lo < __hip_atomic_load(&hrange[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMin(&hrange[0], lo);
-      if (hi > __hip_atomic_load(&hrange[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
-        atomicMax(&hrange[1], hi);
-    }
+  // uniformity of h: hrange starts as (h_0, h_0) (hrange_init); a particle
+  // whose h differs stores its own h into hrange[1] (any such store makes
+  // the pair unequal), so uniform h costs no store at all
+  if (hrange != nullptr && valid) {
+    const int hb = __float_as_int(a.h[i]);
+    if (hb != __float_as_int(a.h[0])) hrange[1] = hb;
   }
 }
 
-__global__ void hrange_init(int* __restrict__ hrange) {
-  hrange[0] = 0x7f7fffff;
-  hrange[1] = 0;
+__global__ void hrange_init(const float* __restrict__ h, int* __restrict__ hrange) {
+  hrange[0] = __float_as_int(h[0]);
+  hrange[1] = __float_as_int(h[0]);
 }
 
 // Compile-time dimension of the FP32-pair scan kernels (0: from the
@@ -341,7 +319,8 @@ extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, voi
   }
   if (a->n == 0) return SPH_OK;
   hipStream_t st = as_stream(stream);
-  if (hrange_dev != nullptr) hipLaunchKernelGGL(hrange_init, dim3(1), dim3(1), 0, st, hrange_dev);
+  if (hrange_dev != nullptr)
+    hipLaunchKernelGGL(hrange_init, dim3(1), dim3(1), 0, st, (const float*)a->h, hrange_dev);
   hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, st,
                      to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev,
                      (float4*)pm4_dev, (int*)hrange_dev);

@@ -268,8 +268,11 @@ class WCSPHPlan:
         v = pa.view()
         for f, t in (fields or {}).items():
             setattr(v, f, ptr(t))
+        # the stage-2 repack (fields: X1 positions and velocities) keeps h,
+        # so the h-uniformity pair of the step's first pack stays valid
+        hr = None if fields else pk["hrange"].data_ptr()
         _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr(),
-                  pk["pos3"].data_ptr(), pk["pm4"].data_ptr(), pk["hrange"].data_ptr())
+                  pk["pos3"].data_ptr(), pk["pm4"].data_ptr(), hr)
         pk["version"] = ver
 
     def _pack_all(self):
