@@ -7,6 +7,7 @@ FP64 storage within 1e-12.  Inputs are the same stored bits on both sides.
 """
 import numpy as np
 import pytest
+import torch
 
 import pysph_amd.base.api as base
 import pysph_amd.sph.api as sph
@@ -81,6 +82,90 @@ def test_sphrho_sources_of_uniform_and_varied_h(case):
         ref = O.sphrho(2, to_oracle(dst), [to_oracle(s) for s in srcs],
                        cell_size=particles.cell_manager.cell_size)
         assert max_rel(dst.get("rho"), ref) < TOL["single"]
+
+
+def _tag(pa, frac, value, seed):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = pa.get_number_of_particles()
+    t = np.where(rng.random(n) < frac, value, base.LocalReal).astype(np.int64)
+    pa.set(tag=t)
+    return t
+
+
+def _aux4(pa):
+    """(rho, cs, p/rho^2, p) per slot as the density pass writes it (p/rho^2
+    formed in FP64 from the stored FP32 values)."""
+    n = pa.get_number_of_particles()
+    rho = pa.get_carray("rho")[:n].double()
+    p = pa.get_carray("p")[:n].double()
+    cs = pa.get_carray("cs")[:n].double()
+    return torch.stack([rho, cs, p / (rho * rho), p], 1).float().contiguous()
+
+
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_non_local_destinations_get_zero(precision):
+    """Destinations that are not LocalReal get 0 in every output and still
+    act as sources (sph_func.pyx:213-219): 10 % of the fluid RemoteReal and
+    10 % of the boundary LocalDummy.  Momentum + XSPH through the SoA kernels
+    and (FP32) the packed kernels, then SPHRho + Tait, each against the
+    oracle with the same tags."""
+    f, b, info = _dam2d(precision, seed=5)
+    _randomise_state(f, 6, info, 2)
+    _eos(f, info, 0)
+    _eos(b, info, 1)
+    tf = _tag(f, 0.1, base.RemoteReal, 7)
+    tb = _tag(b, 0.1, base.LocalDummy, 8)
+    assert 0 < np.count_nonzero(tf) < len(tf) and 0 < np.count_nonzero(tb) < len(tb)
+    particles = base.Particles([f, b])
+    cm = particles.cell_manager
+    cm.update()
+    rf, rb = to_oracle(f), to_oracle(b)
+    tol = TOL[precision]
+
+    # momentum + XSPH on the preset state
+    prm = _ops.make_params(2, alpha=0.1, beta=0.0, eta=0.1, eps=0.5,
+                           pair_prec=_abi.SPH_PAIR_F64)
+    flags = [_abi.SPH_SRC_MOMENTUM | _abi.SPH_SRC_XSPH, _abi.SPH_SRC_MOMENTUM]
+    a_o = O.momentum(2, rf, [rf, rb], 0.1, 0.0, 0.1, cell_size=cm.cell_size)[:2]
+    d_o = O.xsph(2, rf, [rf], 0.5, cell_size=cm.cell_size)[:2]
+    assert np.all(a_o[:, tf != 0] == 0) and np.all(d_o[:, tf != 0] == 0)
+    order = f._order_np()
+    n = f.get_number_of_particles()
+    overrides = [None]
+    if precision == "single":
+        ov = _ops.packed_override([f, b])
+        for pa in (f, b):
+            ov[id(pa)]["aux4"] = _aux4(pa)
+        overrides.append(ov)
+    for ov in overrides:
+        acc = [f.get_carray("u").new_full((n,), 7.0) for _ in range(3)]
+        dv = [f.get_carray("u").new_full((n,), 7.0) for _ in range(3)]
+        _ops.momentum_xsph(cm, prm, f, [f, b], flags, acc=acc, dv=dv, override=ov)
+        def user(t):
+            a = t.cpu().numpy().astype(np.float64)
+            out = np.empty_like(a)
+            out[order] = a
+            return out
+        a_g = np.stack([user(t) for t in acc[:2]])
+        d_g = np.stack([user(t) for t in dv[:2]])
+        assert np.all(a_g[:, tf != 0] == 0) and np.all(d_g[:, tf != 0] == 0)
+        assert rel_norm(a_g, a_o) < tol
+        assert rel_norm(d_g, d_o) < tol
+
+    # SPHRho + Tait: rho, p and cs of non-local destinations are 0
+    prm = _ops.make_params(2, co=info["co"], ro=info["rho0"], gamma=7.0)
+    for dst, tags, ref in ((f, tf, rf), (b, tb, rb)):
+        rho_o = O.sphrho(2, ref, [rf, rb], cell_size=cm.cell_size)
+        assert np.all(rho_o[tags != 0] == 0)
+        _ops.density(cm, prm, dst, [f, b], dst.get_carray("rho"), dst.get_carray("p"),
+                     dst.get_carray("cs"))
+        rho = dst.get("rho")
+        assert np.all(rho[tags != 0] == 0)
+        assert max_rel(rho, rho_o, floor=1.0) < tol
+        p_o, cs_o = O.tait(to_oracle(dst), info["co"], info["rho0"], 7.0)
+        for got, want in ((dst.get("p"), p_o), (dst.get("cs"), cs_o)):
+            assert np.all(got[tags != 0] == 0)
+            assert max_rel(got, want, floor=float(np.max(np.abs(want)))) < 1e-6
 
 
 @pytest.mark.parametrize("precision", ["single", "double"])
