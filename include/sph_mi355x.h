@@ -127,7 +127,13 @@ typedef struct sph_params {
      * The neighbour passes run their SoA kernels in this mode (packed
      * inputs and neighbour lists are ignored); the batched engine rejects it. */
     int32_t symmetric;
-    int32_t reserved;
+    /* Bit s set: the summation density drops the self pair of source s, which
+     * must be the destination array (SPHFunctionParticle exclude_self,
+     * sph_func.pyx:386-389, where the self particle is the last neighbour).
+     * Read by sph_density / sph_density_cached only: in the other passes
+     * the self pair contributes 0.  The density then runs its SoA kernels;
+     * the batched engine rejects it. */
+    int32_t exclude_self;
 } sph_params;
 
 int32_t sph_abi_version(void);

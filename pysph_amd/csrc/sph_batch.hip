@@ -303,6 +303,10 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
       set_error("sph_batch_create: scene %d: variable-h symmetric pairs are not supported", s);
       return SPH_EUNSUPPORTED;
     }
+    if (S.prm.exclude_self) {
+      set_error("sph_batch_create: scene %d: exclude_self is not supported", s);
+      return SPH_EUNSUPPORTED;
+    }
     f_end += S.nf;
     b_end += S.nb;
     ncells += scene_cells(S);

@@ -78,6 +78,7 @@ struct PassArgs {
   int nl_mode;  // 0 build, 1 use
   float nl_skin;
   int symmetric;  // variable-h symmetric pairs
+  unsigned excl_self;  // density: sources whose self pair is dropped
   T* rate;        // SPHDensityRate output
   // density: stored partial sums of source cache_src (mode 1 fill, 2 use)
   double* cache_sums;
@@ -102,6 +103,7 @@ inline void set_params(PassArgs<T>& a, const sph_params* prm) {
   for (int c = 0; c < 3; ++c) a.g[c] = prm->g[c];
   a.has_gravity = prm->has_gravity;
   a.symmetric = prm->symmetric != 0;
+  a.excl_self = (unsigned)prm->exclude_self;
 }
 
 // Distance to a periodic image source at (sx,sy,sz), FP64 without

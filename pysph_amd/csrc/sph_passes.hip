@@ -67,17 +67,23 @@ __global__ __launch_bounds__(kBlock) void density_kernel(Grid g, PassArgs<T> a) 
     const View<T> S = a.src[s];
     CT nr = CT(0);
     if (s == a.cache_src && a.cache_mode == 2) {
-      rho += (CT)a.cache_sums[i];  // stored partial sum of a static source
-      continue;
+      nr = (CT)a.cache_sums[i];  // stored partial sum of a static source
+    } else {
+      scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
+                        [&](int j, CT d2, CT, CT, CT) {
+                          const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
+                          const CT inv = CT(1) / hab;
+                          const CT q = sqrt(d2) * inv;
+                          nr += (CT)S.m[j] * (w_shape<CT>(q) * h_dim_inv<CT>(inv, dim));
+                        });
+      if (s == a.cache_src && a.cache_mode == 1) a.cache_sums[i] = (double)nr;
     }
-    scan_pairs<T, CT>(g, a, S, xd, yd, zd, cx, cy, cz, ring, r2f, r2d,
-                      [&](int j, CT d2, CT, CT, CT) {
-                        const CT hab = CT(0.5) * ((CT)hd + (CT)S.h[j]);
-                        const CT inv = CT(1) / hab;
-                        const CT q = sqrt(d2) * inv;
-                        nr += (CT)S.m[j] * (w_shape<CT>(q) * h_dim_inv<CT>(inv, dim));
-                      });
-    if (s == a.cache_src && a.cache_mode == 1) a.cache_sums[i] = (double)nr;
+    if ((a.excl_self >> s) & 1u) {
+      // the self pair (d2 = 0, h_ab = h_a) as the scan added it
+      const CT hab = CT(0.5) * ((CT)hd + (CT)hd);
+      const CT inv = CT(1) / hab;
+      nr -= (CT)S.m[i] * (w_shape<CT>(sqrt(CT(0)) * inv) * h_dim_inv<CT>(inv, dim));
+    }
     rho += nr;
   }
   rho *= (CT)a.fac1;
@@ -374,7 +380,7 @@ static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params*
   Grid G = make_grid(*g, rmax);
   if (dst->n == 0) return SPH_OK;
   SPH_TRY(check_period(g, rmax, "sph_density"));
-  if (packed_inputs(a, false) && !a.symmetric && !G.periodic) {
+  if (packed_inputs(a, false) && !a.symmetric && !G.periodic && a.excl_self == 0) {
     if (nl != nullptr) {
       a.nl_idx = nl->idx;
       a.nl_count = nl->count;
@@ -539,6 +545,13 @@ static int32_t density_entry(void* stream, const sph_grid* g, const sph_params* 
   if (nlist_out != nullptr && (!nlist_out->idx || !nlist_out->count || nlist_out->max_per < 0)) {
     set_error("sph_density: neighbour list needs idx, count and max_per >= 0");
     return SPH_EINVAL;
+  }
+  for (int s = 0; s < 32; ++s) {
+    if (!((prm->exclude_self >> s) & 1)) continue;
+    if (s >= nsrc || srcs[s].x != dst->x || srcs[s].n != dst->n) {
+      set_error("sph_density: exclude_self names source %d, which is not the destination", s);
+      return SPH_EINVAL;
+    }
   }
   if (dst->dtype == SPH_F32)
     return density_impl<float>(as_stream(stream), g, prm, dst, nsrc, srcs, rmax, disp_dev,

@@ -148,6 +148,8 @@ class WCSPHPlan:
         n = len(calcs)
         rho_calcs = []
         while i < n and kind(calcs[i]) is SPHRho and not calcs[i].integrates:
+            if any(getattr(f, "exclude_self", False) for f in calcs[i].funcs):
+                return None  # the per-calc path drops the self pairs
             rho_calcs.append(calcs[i])
             i += 1
         tait_calcs = []

@@ -20,9 +20,10 @@ DEFAULT_PAIR_PREC = _abi.SPH_PAIR_NATIVE
 
 def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, beta=1.0,
                 eta=0.1, eps=0.5, g=(0.0, 0.0, 0.0), has_gravity=False, has_xsph=False,
-                pair_prec=None, symmetric=False):
+                pair_prec=None, symmetric=False, exclude_self=0):
     """sph_params; `symmetric` selects the variable-h pair sets of
-    VarHNbrParticleLocator (nnps.pyx:654-749)."""
+    VarHNbrParticleLocator (nnps.pyx:654-749); bit s of `exclude_self`
+    drops the self pair of source s (the destination) from the density."""
     p = _abi.SphParams()
     p.dim = int(dim)
     p.pair_prec = DEFAULT_PAIR_PREC if pair_prec is None else int(pair_prec)
@@ -34,6 +35,7 @@ def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, 
     p.has_gravity = 1 if has_gravity else 0
     p.has_xsph = 1 if has_xsph else 0
     p.symmetric = 1 if symmetric else 0
+    p.exclude_self = int(exclude_self)
     return p
 
 

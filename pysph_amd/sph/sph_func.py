@@ -112,15 +112,16 @@ class _SingleCalc:
 
 class SPHFunctionParticle(SPHFunction):
     """Operators that sum over neighbours (sph_func.pyx:252-403).  hks
-    kernel symmetrisation and exclude_self are not provided."""
+    kernel symmetrisation is not provided.  exclude_self drops the self pair
+    when the source is the destination (sph_func.pyx:386-389); of the
+    functions here only the summation density changes, since the self pair
+    adds 0 to the gradient and velocity-difference sums."""
 
     def __init__(self, source, dest, setup_arrays=True, hks=False, exclude_self=False,
                  nbr_locator=None, **kwargs):
         SPHFunction.__init__(self, source, setup_arrays=False, **kwargs)
         if hks:
             raise NotImplementedError("hks kernel symmetrisation is not supported")
-        if exclude_self:
-            raise NotImplementedError("exclude_self is not supported")
         self.dest = dest
         self.exclude_self = exclude_self
         self.bonnet_and_lok_correction = False

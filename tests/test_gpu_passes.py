@@ -84,6 +84,54 @@ def test_sphrho_sources_of_uniform_and_varied_h(case):
         assert max_rel(dst.get("rho"), ref) < TOL["single"]
 
 
+@pytest.mark.parametrize("precision", ["single", "double"])
+def test_sphrho_exclude_self(precision):
+    """exclude_self (sph_func.pyx:386-389) drops the self pair of the source
+    that is the destination and leaves other sources alone: the fluid sum
+    is the full oracle sum less m_a W(0, h_a) = m_a 10/(7 pi h_a^2) (2D),
+    and a source that is not the destination cannot be flagged."""
+    f, b, info = _dam2d(precision, seed=2)
+    particles = base.Particles([f, b])
+    rho_f = sph.SPHRho.withargs(exclude_self=True)
+    calc = sph.SPHCalc(particles, [f, b], f, base.CubicSplineKernel(dim=2),
+                       [rho_f.get_func(f, f), rho_f.get_func(b, f)], updates=["rho"])
+    calc.sph("rho")
+    cm = particles.cell_manager
+    full = O.sphrho(2, to_oracle(f), [to_oracle(f), to_oracle(b)], cell_size=cm.cell_size)
+    h = f.get("h").astype(np.float64)
+    w0 = f.get("m").astype(np.float64) * 10.0 / (7.0 * np.pi * h * h)
+    assert max_rel(f.get("rho"), full - w0) < TOL[precision]
+    # the boundary as destination: its own self pair goes, the fluid source
+    # is not the destination and keeps every pair
+    calc_b = sph.SPHCalc(particles, [f, b], b, base.CubicSplineKernel(dim=2),
+                         [rho_f.get_func(f, b), rho_f.get_func(b, b)], updates=["rho"])
+    calc_b.sph("rho")
+    full_b = O.sphrho(2, to_oracle(b), [to_oracle(f), to_oracle(b)], cell_size=cm.cell_size)
+    hb = b.get("h").astype(np.float64)
+    w0b = b.get("m").astype(np.float64) * 10.0 / (7.0 * np.pi * hb * hb)
+    assert max_rel(b.get("rho"), full_b - w0b) < TOL[precision]
+    prm = _ops.make_params(2, exclude_self=0b10)  # source 1 (boundary) is not f
+    with pytest.raises(Exception, match="exclude_self"):
+        _ops.density(cm, prm, f, [f, b], f.get_carray("rho"))
+
+
+def test_reference_four_particles_exclude_self():
+    """The unit square of density_funcs' test (h = 1, m = 1) without the
+    self term: rho = fac (2 W(1) + W(sqrt 2)) = fac (0.5 + 0.25 (2 - sqrt 2)^3)."""
+    x = np.array([0, 0, 1, 1.0])
+    y = np.array([0, 1, 1, 0.0])
+    pa = base.get_particle_array(name="test", x=x, y=y, z=np.zeros(4), m=np.ones(4))
+    particles = base.Particles([pa])
+    calc = sph.SPHCalc(particles, [pa], pa, base.CubicSplineKernel(dim=2),
+                       [sph.SPHRho.withargs(exclude_self=True).get_func(pa, pa)],
+                       updates=["rho"])
+    pa.add_property({"name": "_tmpx"})
+    calc.sph("_tmpx")
+    fac = 10 / (7 * np.pi)
+    assert np.allclose(pa.get("_tmpx"), fac * (0.5 + 0.25 * (2 - np.sqrt(2)) ** 3),
+                       rtol=0, atol=1e-14)
+
+
 def _tag(pa, frac, value, seed):
     rng = np.random.Generator(np.random.PCG64(seed))
     n = pa.get_number_of_particles()
