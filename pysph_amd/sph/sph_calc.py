@@ -80,7 +80,9 @@ class SPHCalc:
 
     def sph(self, output_array1=None, output_array2=None, output_array3=None,
             exclude_self=False):
-        """Evaluate into the named destination properties."""
+        """Evaluate into the named destination properties.  `exclude_self` is
+        accepted and ignored, as in sph_calc.pyx:244-268; the functions' own
+        exclude_self drops the self pair."""
         names = [n for n in (output_array1, output_array2, output_array3) if n is not None]
         # bin first: a re-binning permutes (replaces) the property tensors
         if self.nbr_info:
@@ -96,7 +98,8 @@ class SPHCalc:
             self.particles.update()
 
     def sph_array(self, output1, output2=None, output3=None, exclude_self=False):
-        """Evaluate into device tensors instead of named properties."""
+        """Evaluate into device tensors instead of named properties
+        (`exclude_self` ignored, as in sph())."""
         outs = [o for o in (output1, output2, output3) if o is not None]
         if self.nbr_info:
             self.cell_manager.update()
