@@ -268,3 +268,35 @@ def test_static_source_cache_gives_same_bits(precision, packed):
             assert plan._caches, "the boundary density calc keeps a cache"
     for c in out[0]:
         assert np.array_equal(out[0][c], out[1][c]), c
+
+
+def test_exclude_self_density_runs_calc_by_calc():
+    """An operation set whose SPHRho has exclude_self is not the fused
+    plan's: the integrator runs it calc by calc, and one Euler step leaves
+    each array's rho at the oracle's sum over the step's start positions less
+    m_a W(0, h_a) = m_a 10/(7 pi h_a^2) (sph_func.pyx:386-389)."""
+    import pysph_amd.sph.api as sph
+    f, b, info = gen.dam_break_2d(width=0.2, height=0.3, dx=0.01, jitter=0.02, seed=1)
+    particles = base.Particles([f, b])
+    s = solver.Solver(dim=2, integrator_type=solver.EulerIntegrator)
+    types = [base.Fluid, base.Solid]
+    s.add_operation(solver.SPHOperation(sph.SPHRho.withargs(exclude_self=True), on_types=types,
+                                        from_types=types, updates=["rho"], id="density"))
+    s.add_operation(solver.SPHOperation(
+        sph.TaitEquation.withargs(co=info["co"], ro=info["rho0"], gamma=7.0), on_types=types,
+        updates=["p", "cs"], id="eos"))
+    s.add_operation(solver.SPHIntegration(
+        sph.MomentumEquation.withargs(alpha=0.1, beta=0.0, eta=0.1, dim=2),
+        on_types=[base.Fluid], from_types=types, updates=["u", "v"], id="mom"))
+    s.add_operation_step([base.Fluid])
+    s.setup_integrator(particles)
+    assert s.integrator.plan is None
+    rf, rb = to_oracle(f), to_oracle(b)
+    cs = particles.cell_manager.cell_size
+    full = {id(f): O.sphrho(2, rf, [rf, rb], cell_size=cs),
+            id(b): O.sphrho(2, rb, [rf, rb], cell_size=cs)}
+    s.integrator.integrate(info["dt"])
+    for pa in (f, b):
+        h = pa.get("h").astype(np.float64)
+        w0 = pa.get("m").astype(np.float64) * 10.0 / (7.0 * np.pi * h * h)
+        assert max_rel(pa.get("rho"), full[id(pa)] - w0) < 1e-5, pa.name
