@@ -270,10 +270,15 @@ typedef struct sph_nlist {
 /* SPHRho over all sources (density_funcs.pyx:38-84), optionally followed by
  * TaitEquation on the destination (eos_funcs.pyx:105-116) when p_dev and
  * cs_dev are both non-NULL.  With packed inputs the destination's aux4
- * (when set) receives (rho, cs, p/rho^2, p).  `disp_dev` (nullable, one float) is the largest
- * coordinate displacement of any particle since the arrays were binned; the
- * stencil ring grows to cover it so that a binning can be reused across an
- * RK2 stage.  rmax = the largest search radius (radius_scale * max h). */
+ * (when set) receives (rho, cs, p/rho^2, p).  `disp_dev` (nullable, one float) bounds the
+ * straight-line displacement |x_now - x_binned| of every particle since the
+ * arrays were binned; the stencil ring and the culled search radius grow by
+ * it so that a binning can be reused across an RK2 stage.  rmax = the
+ * largest search radius (radius_scale * max h).  `nlist_out` (nullable)
+ * records the accepted sources; lists are written by the packed FP32 path
+ * only, so a call that takes the SoA kernels (FP64 storage, no packed
+ * inputs, symmetric pairs, periodic axes or exclude_self) with a non-NULL
+ * `nlist_out` returns SPH_EINVAL instead of leaving the list unwritten. */
 int32_t sph_density(void *stream, const sph_grid *g, const sph_params *prm,
                     const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                     double rmax, const float *disp_dev, void *rho_dev,
@@ -340,8 +345,12 @@ int32_t sph_momentum_xsph(void *stream, const sph_grid *g, const sph_params *prm
  * `dst` is the state the right-hand side is evaluated at; state0_dev[6]
  * holds x0,y0,z0,u0,v0,w0; out_dev[6] receives the new state (it may alias
  * state0_dev only for stage 2, whose pair sums read `dst` and `srcs`, never
- * state0).  disp_out_dev (nullable) receives the max |out.x - x0| component
- * via atomic max (for the stage-2 ring), it must be zeroed by the caller. */
+ * state0).  disp_out_dev (nullable) receives, via atomic max, an upper bound
+ * of the largest straight-line move |out.x - x0| (Euclidean over the dim
+ * components; the stage-2 passes widen their ring and culled search radius
+ * by it); it must be zeroed by the caller.  `nlist_in` (nullable) is read
+ * by the packed FP32 path only: a call that takes the SoA kernels with a
+ * non-NULL `nlist_in` returns SPH_EINVAL. */
 int32_t sph_wcsph_stage(void *stream, int32_t stage, const sph_grid *g,
                         const sph_params *prm, const sph_view *dst, int32_t nsrc,
                         const sph_view *srcs, const uint32_t *src_flags_host,
@@ -362,8 +371,9 @@ int32_t sph_wcsph_stage(void *stream, int32_t stage, const sph_grid *g,
  * mode 5: out = (a + 2b) + 2c            (RK4 partial sum k1 + 2k2 + 2k3, :669-689)
  * mode 7: out = 2a - b                    (PredictorCorrector final_step, :756-772)
  * mode 8: out = a + s*(b - c)            (LeapFrog final_step, s = 0.5*dt, :866-883)
- * With tag_dev non-NULL, modes 3 and 4 write 0 where tag != LocalReal
- * (sph_func.pyx:211-234).  nops arrays of n elements each, one dtype. */
+ * With tag_dev non-NULL, modes 0, 3 and 4 write 0 where tag != LocalReal
+ * (sph_func.pyx:211-234: SPHFunction.eval accumulates into the outputs of
+ * LocalReal destinations and sets the others to 0).  nops arrays of n elements each, one dtype. */
 typedef struct sph_update_op {
     void *out;
     void *a;

@@ -391,8 +391,10 @@ class CellManager:
         cs = self.cell_size
         per = {}
         for pa in self.arrays_to_bin:
-            x, y, z = (pa.get(c).astype(np.float64) for c in ("x", "y", "z"))
-            idx = pa.get("idx").astype(np.int64) if pa.has_array("idx") else \
+            x, y, z = (pa.get(c, only_real_particles=False).astype(np.float64)
+                       for c in ("x", "y", "z"))
+            idx = pa.get("idx", only_real_particles=False).astype(np.int64) \
+                if pa.has_array("idx") else \
                 np.arange(len(x), dtype=np.int64)
             cid = np.stack([np.floor(a / cs) for a in (x, y, z)], 1).astype(np.int32)
             per[pa.name] = (x, y, z, idx, cid)

@@ -43,6 +43,17 @@ __device__ __forceinline__ int clampi(int v, int lo, int hi) {
   return v < lo ? lo : (v > hi ? hi : v);
 }
 
+// Upper bound, in FP32, of a particle's straight-line displacement from its
+// per-axis moves summed in squares (mv2, FP64).  A kept binning is reused
+// with every search radius widened by the largest such bound: a source can
+// lie that far (in any direction, not only along one axis) from the cell it
+// was binned in, so the straight-line bound, not the largest per-axis move,
+// keeps the culled stencil exact.  It also bounds every per-axis move, which
+// is what the ring below needs.
+__device__ __forceinline__ float move_bound(double mv2) {
+  return (float)(sqrt(mv2) * (1.0 + 1e-6));
+}
+
 // Stencil ring: ceil(r/cs) cells (cell.pyx:1127-1129) widened by the largest
 // displacement since binning, so that a binning stays exact when reused.
 __device__ __forceinline__ int ring_of(const Grid& g, const float* disp) {

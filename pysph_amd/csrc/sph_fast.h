@@ -548,7 +548,7 @@ __device__ __forceinline__ void mom_epilogue(const PassArgs<float>& a, int i, bo
   const float dt = (float)a.dt;
   const float hdt = (float)(0.5 * a.dt);
   const float vel[3] = {vd.x, vd.y, vd.z};
-  float disp = 0.f;
+  double mv2 = 0.0;
   for (int c = 0; c < a.dim; ++c) {
     const float x0 = a.s0[c][i];
     const float u0 = a.s0[3 + c][i];
@@ -575,9 +575,11 @@ __device__ __forceinline__ void mom_epilogue(const PassArgs<float>& a, int i, bo
     }
     a.out[c][i] = xn;
     a.out[3 + c][i] = un;
-    disp = fmaxf(disp, fabsf(xn - x0));
+    const double mv = (double)xn - (double)x0;  // exact for two floats this close
+    mv2 += mv * mv;
   }
   if (a.disp_out != nullptr) {
+    float disp = move_bound(mv2);
     for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
     if ((threadIdx.x & 63) == 0) atomicMax((int*)a.disp_out, __float_as_int(disp));
   }

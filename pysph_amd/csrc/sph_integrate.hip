@@ -31,7 +31,10 @@ __global__ __launch_bounds__(kBlock) void update_kernel(UpdArgs<T> u) {
   if (i >= u.n) return;
   for (int k = 0; k < u.nops; ++k) {
     T r;
-    if (MODE == 0 || MODE == 2) {
+    if (MODE == 0) {
+      // SPHFunction.eval: accumulate, 0 where tag != LocalReal
+      r = (u.tag == nullptr || u.tag[i] == 0) ? u.a[k][i] + u.b[k][i] * u.s : T(0);
+    } else if (MODE == 2) {
       r = u.a[k][i] + u.b[k][i] * u.s;
     } else if (MODE == 1) {
       r = u.a[k][i] + u.s * (u.b[k][i] + u.c[k][i]);

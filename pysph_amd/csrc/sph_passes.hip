@@ -251,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
   const T dt = (T)a.dt;
   const T hdt = (T)(0.5 * a.dt);
   const T vel[3] = {ud, vd, wd};
-  float disp = 0.f;
+  double mv2 = 0.0;
   for (int c = 0; c < dim; ++c) {
     const T x0 = a.s0[c][i];
     const T u0 = a.s0[3 + c][i];
@@ -278,10 +278,12 @@ __global__ __launch_bounds__(kBlock) void momentum_kernel(Grid g, PassArgs<T> a)
     }
     a.out[c][i] = xn;
     a.out[3 + c][i] = un;
-    disp = fmaxf(disp, fabsf((float)(xn - x0)));
+    const double mv = (double)xn - (double)x0;
+    mv2 += mv * mv;
   }
   if (a.disp_out != nullptr) {
     // wave max, then one atomic per wave (non-negative floats order as ints)
+    float disp = move_bound(mv2);
     for (int off = 32; off > 0; off >>= 1) disp = fmaxf(disp, __shfl_xor(disp, off, 64));
     if ((threadIdx.x & 63) == 0) atomicMax((int*)a.disp_out, __float_as_int(disp));
   }
@@ -391,6 +393,11 @@ static int32_t density_impl(hipStream_t st, const sph_grid* g, const sph_params*
     return fast_density(st, *g, rmax, reinterpret_cast<const PassArgs<float>&>(a), do_tait,
                         prm->pair_prec);
   }
+  if (nl != nullptr) {
+    set_error("sph_density: neighbour lists are written by the packed FP32 path only (no "
+              "FP64 storage, symmetric pairs, periodic axes or exclude_self)");
+    return SPH_EINVAL;
+  }
   if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
     launch_density<T, double>(st, G, a, do_tait);
   else
@@ -472,6 +479,11 @@ static int32_t momentum_impl(hipStream_t st, int mode, const sph_grid* g, const 
     }
     return fast_momentum(st, *g, rmax, reinterpret_cast<const PassArgs<float>&>(a), mode,
                          prm->pair_prec);
+  }
+  if (nl != nullptr) {
+    set_error("%s: neighbour lists are read by the packed FP32 path only (no FP64 storage, "
+              "symmetric pairs or periodic axes)", who);
+    return SPH_EINVAL;
   }
   if (sizeof(T) == 4 && prm->pair_prec == SPH_PAIR_F64)
     launch_momentum<T, double>(st, G, a, mode);

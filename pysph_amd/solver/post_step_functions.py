@@ -40,11 +40,12 @@ def neighbor_record(loc, cell_size):
     """The dump arrays of one locator (rows in the destination's user order)."""
     off, nb = loc.get_csr()
     dest, src = loc.dest, loc.source
-    didx = dest.get("idx").astype(np.int64) if dest.has_array("idx") else \
+    every = {"only_real_particles": False}  # one row per destination particle
+    didx = dest.get("idx", **every).astype(np.int64) if dest.has_array("idx") else \
         np.arange(dest.get_number_of_particles(), dtype=np.int64)
-    sidx = src.get("idx").astype(np.int64) if src.has_array("idx") else \
+    sidx = src.get("idx", **every).astype(np.int64) if src.has_array("idx") else \
         np.arange(src.get_number_of_particles(), dtype=np.int64)
-    x, y, z = dest.get("x", "y", "z")
+    x, y, z = dest.get("x", "y", "z", **every)
     return {"idx": didx, "offsets": np.asarray(off, dtype=np.int64),
             "neighbors": sidx[np.asarray(nb, dtype=np.int64)],
             "cid": cell_ids(x, y, z, cell_size)}

@@ -206,9 +206,10 @@ class Solver:
                 props = pa.properties
                 np.savez(path, dt=self.dt, **props)
             else:
+                # get() returns the LocalReal particles, np counts them
                 props = {p: pa.get(p) for p in print_properties if pa.has_array(p)}
                 np.savez(path, dt=self.dt, cell_size=cell_size,
-                         np=pa.get_number_of_particles(), **props)
+                         np=pa.num_real_particles, **props)
             paths.append(path)
         return paths
 

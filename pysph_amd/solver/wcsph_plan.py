@@ -413,7 +413,13 @@ class WCSPHPlan:
         b["s1"] = moved[:6]
         b["k1"] = moved[6:]
 
-    def rk2_step(self, dt):
+    def rk2_step(self, dt, probe=None):
+        """One RK2 step.  `probe` (optional) is called as probe(phase, bufs,
+        override) after each phase ("density1", "stage1", "density2",
+        "stage2") with the stage buffers and the field overrides the next
+        pass reads, so that a test can inspect the intermediate sums."""
+        if probe is None:
+            probe = _no_probe
         self.cm.update()
         b = self._buffers()
         self._pack_all()
@@ -432,23 +438,31 @@ class WCSPHPlan:
             stream = require_gpu(b["disp"])
             _abi.call("sph_fill_i32", stream, b["disp"].data_ptr(), 1, 0)
             self._density(override=ov, list_mode=0)
+            probe("density1", b, ov)
             self._stage(1, dt, b, override=ov, disp_out=b["disp"])
+            probe("stage1", b, ov)
             s1 = self._s1_fields(b)
             self._pack(self.fluid, fields=s1, force=True)
             ov = self._override(s1)
             self._density(disp=b["disp"], override=ov, list_mode=1)
+            probe("density2", b, ov)
             self._stage(2, dt, b, override=ov, disp_in=b["disp"])
+            probe("stage2", b, ov)
         else:
             # each stage bins its own positions; with lists, the density pass
             # of each stage records the neighbours its momentum pass reads
             self._density(override=ov, list_mode=0)
+            probe("density1", b, ov)
             self._stage(1, dt, b, override=ov)
+            probe("stage1", b, ov)
             self.rebin_stage(b)
             s1 = self._s1_fields(b)
             self._pack(self.fluid, fields=s1, force=True)
             ov = self._override(s1)
             self._density(override=ov, list_mode=0)
+            probe("density2", b, ov)
             self._stage(2, dt, b, override=ov)
+            probe("stage2", b, ov)
         self.fluid.set_dirty(True)
         self.steps += 1
 
@@ -469,3 +483,7 @@ class WCSPHPlan:
                 b["s1"][c] = old
         pa.set_dirty(True)
         self.steps += 1
+
+
+def _no_probe(phase, bufs, override):
+    pass

@@ -84,7 +84,9 @@ class SPHFunction:
 
     def eval(self, kernel, output1=None, output2=None, output3=None):
         """Accumulate this function's contribution from its one source into
-        the outputs (device tensors of the destination, slot order)."""
+        the outputs (device tensors of the destination, slot order) of
+        LocalReal destinations, and set the outputs of the other destinations
+        to 0 (sph_func.pyx:199-234)."""
         outs = [o for o in (output1, output2, output3) if o is not None][:self.num_outputs]
         if self.kernel is None:
             self.kernel = kernel
@@ -92,7 +94,8 @@ class SPHFunction:
         calc = _SingleCalc(self, kernel)
         type(self).calc_eval(calc, tmp)
         n = self.dest.get_number_of_particles()
-        _ops.update(self.dest.dtype, n, 0, 1.0, [(o, o, t, None) for o, t in zip(outs, tmp)])
+        _ops.update(self.dest.dtype, n, 0, 1.0, [(o, o, t, None) for o, t in zip(outs, tmp)],
+                    tag=self.dest.get_carray("tag"))
 
 
 class _SingleCalc:
@@ -112,10 +115,18 @@ class _SingleCalc:
 
 class SPHFunctionParticle(SPHFunction):
     """Operators that sum over neighbours (sph_func.pyx:252-403).  hks
-    kernel symmetrisation is not provided.  exclude_self drops the self pair
-    when the source is the destination (sph_func.pyx:386-389); of the
-    functions here only the summation density changes, since the self pair
-    adds 0 to the gradient and velocity-difference sums."""
+    kernel symmetrisation is not provided.
+
+    exclude_self drops the self pair when the source is the destination, the
+    intent of sph_func.pyx:386-389.  That reference code tests `self.src`,
+    an attribute SPHFunctionParticle does not declare: the AttributeError is
+    swallowed by the `cdef void eval_single` before `result` is zeroed, so
+    the reference adds an unset result and no reference output exists for
+    this option.  This class implements the intended semantics on purpose
+    (pinned by the closed-form self term m_a W(0, h_a), not by the
+    reference).  Of the functions here only the summation density changes,
+    since the self pair adds 0 to the gradient and velocity-difference
+    sums."""
 
     def __init__(self, source, dest, setup_arrays=True, hks=False, exclude_self=False,
                  nbr_locator=None, **kwargs):

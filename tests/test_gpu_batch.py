@@ -116,3 +116,80 @@ def test_autotune_then_graph_replays_match_eager():
                 for c in FIELDS:
                     assert np.array_equal(a[c], b_[c]), (rep, k, c)
         eng.close()
+
+
+def _oracle_scene(f, b, info, nsteps):
+    """The FP64 oracle's RK2 steps (integrator.py:581-645 per scene) from the
+    scene's stored FP32 bits."""
+    from oracle import oracle as O
+    from tests.util import to_oracle
+    arrs = [to_oracle(f, 0)] + ([to_oracle(b, 1)] if b is not None else [])
+    sch = O.scheme(2, integrator="rk2", dt=info["dt"], co=info["co"], ro=info["rho0"],
+                   gamma=7.0, alpha=info["alpha"], beta=info["beta"], eta=info["eta"],
+                   gravity=info["g"], xsph_eps=info["eps"] or None, cell_size=2.0 * info["h"])
+    O.scheme_step(sch, arrs, nsteps)
+    return arrs
+
+
+def _check_scene_vs_oracle(batch, k, f, b, info, nsteps):
+    from tests.util import max_rel, rel_norm
+    ref = _oracle_scene(f, b, info, nsteps)
+    got = batch.scene_state(k, FIELDS)
+    scale = max(1.0, float(np.max(np.abs(ref[0]["x"]))), float(np.max(np.abs(ref[0]["y"]))))
+    for c in ("x", "y"):
+        # FP32 storage of the coordinates: a few units in the last place
+        assert np.max(np.abs(got[c] - ref[0][c])) < 5e-7 * scale, (k, info["kind"], c)
+    ev = rel_norm(np.stack([got["u"], got["v"]]), np.stack([ref[0]["u"], ref[0]["v"]]))
+    er = max_rel(got["rho"], ref[0]["rho"])
+    assert ev < 1e-5 and er < 1e-5, (k, info["kind"], ev, er)
+    if b is not None:
+        assert max_rel(batch.boundary_state(k, ("rho",))["rho"], ref[1]["rho"]) < 1e-5
+    return ev, er
+
+
+def test_batch_scenes_vs_oracle():
+    """Scenes of one batch (elliptical drops and dam breaks of 1k-16k
+    particles) against the FP64 oracle, 3 RK2 steps from the same stored
+    bits: density max relative and velocity normwise within 1e-5."""
+    nsteps = 3
+    sc = _scenes(8, seed=11, n_min=1000, n_max=16000)
+    assert {i["kind"] for _, _, i in sc} == {"drop", "dam"}
+    batch = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
+    batch.run(nsteps)
+    torch.cuda.synchronize()
+    for k, (f, b, info) in enumerate(sc):
+        ev, er = _check_scene_vs_oracle(batch, k, f, b, info, nsteps)
+        print("scene %d (%s, %d fluid): velocity %.2e, rho %.2e"
+              % (k, info["kind"], f.get_number_of_particles(), ev, er))
+
+
+def test_full_size_batch_vs_oracle():
+    """The benchmarked batch itself (512 scenes, 1k-64k fluid particles
+    each, one HIP graph per step, autotuned schedules), 3 RK2 steps; the
+    largest and the smallest scene of each kind against the FP64 oracle."""
+    from oracle import oracle as O
+    nsteps = 3
+    sc = gen.batch_scenes(512, seed=0)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        eng = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
+        eng.autotune(reps=1)
+        eng.capture()
+        for _ in range(nsteps):
+            eng.step()
+        torch.cuda.synchronize()
+    picks = []
+    for kind in ("drop", "dam"):
+        ks = [k for k, (_, _, i) in enumerate(sc) if i["kind"] == kind]
+        ks.sort(key=lambda k: sc[k][0].get_number_of_particles())
+        picks += [ks[0], ks[-1]]
+    O.set_threads(0)
+    try:
+        for k in picks:
+            f, b, info = sc[k]
+            ev, er = _check_scene_vs_oracle(eng, k, f, b, info, nsteps)
+            print("scene %d (%s, %d fluid): velocity %.2e, rho %.2e"
+                  % (k, info["kind"], f.get_number_of_particles(), ev, er))
+    finally:
+        O.set_threads(1)
+        eng.close()

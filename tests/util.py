@@ -8,9 +8,9 @@ FIELDS = ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs")
 def to_oracle(pa, ptype=None):
     """FP64 copy (user order) of a ParticleArray for the oracle."""
     from oracle import oracle as O
-    d = {f: pa.get(f).astype(np.float64) for f in FIELDS}
+    d = {f: pa.get(f, only_real_particles=False).astype(np.float64) for f in FIELDS}
     if pa.has_array("tag"):
-        d["tag"] = pa.get("tag").astype(np.int64)
+        d["tag"] = pa.get("tag", only_real_particles=False).astype(np.int64)
     return O.make_array(ptype=pa.particle_type if ptype is None else ptype, **d)
 
 
