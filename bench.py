@@ -26,10 +26,14 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--workload", default="dam3d",
                     choices=["dam3d", "cube-uniform", "cube-clustered", "batch2d",
-                             "surrogate2d"],
+                             "surrogate2d", "c1"],
                     help="dam3d: the 3D dam break step (default); cube-*: neighbour search + "
                          "summation density on 2^24 particles in the unit cube; batch2d: 512 "
-                         "2D scenes (drops and dam breaks) stepped together")
+                         "2D scenes (drops and dam breaks) stepped together; c1: the FP64 2D "
+                         "elliptical drop (5025 particles) through the default solver objects")
+    ap.add_argument("--c1-integrator", choices=["rk2", "euler"], default="rk2",
+                    help="c1: RK2 (stable over the 100 steps) or the specified Euler, whose "
+                         "free surface trips the jump tolerance near step 35 at this CFL")
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--surrogate-n", type=int, default=60000,
                     help="surrogate2d: fluid particles of the 2D dam break")
@@ -603,6 +607,132 @@ def batch_cpu_baseline(sc):
                       "single thread, %.1f s" % (used, total_t)}
 
 
+def main_c1(args):
+    """The 2D elliptical drop of the reference's example (5025 particles on
+    a dx = 0.025 lattice in the unit circle, h = 1.3 dx, m = dx^2, rho0 = 1,
+    u = -100 x, v = 100 y, Tait c0 = 1400, gamma = 7, alpha = 0.01),
+    FP64 storage, dt = 1e-5, through the default Particles / Solver objects
+    (the drop-in path: bounds and jump check read back at every binning).
+    steps/s over `steps` steps after `warmup` steps."""
+    import torch
+    import pysph_amd.base.api as base
+    import pysph_amd.solver.api as solver
+    from pysph_amd.solver import generators as gen
+    from pysph_amd import _abi
+    nsteps = args.steps if args.steps != 1000 else 100
+    dt = 1e-5
+    pa = gen.elliptical_drop(dx=0.025, precision="double")
+    n = pa.get_number_of_particles()
+    particles = base.Particles([pa])
+    s = solver.wcsph_solver(2, integrator=args.c1_integrator, co=1400.0, ro=1.0, gamma=7.0,
+                            alpha=0.01, beta=0.0, eta=0.1, with_solid=False)
+    s.setup_integrator(particles)
+    s.set_time_step(dt)
+    assert s.integrator.plan is not None, "fused plan not selected"
+    cm = particles.cell_manager
+    cm.update()
+    cnt = torch.empty(n, dtype=torch.int64, device=pa.device)
+    _abi.call("sph_neighbor_counts", torch.cuda.current_stream().cuda_stream, cm.grid,
+              pa.view(), pa.view(), 2.0, cnt.data_ptr())
+    pairs0 = int(cnt.sum().item())
+    c0 = s.conservation()
+    for _ in range(args.warmup):
+        s.integrator.integrate(dt)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    e0.record()
+    for _ in range(nsteps):
+        s.integrator.integrate(dt)
+    e1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    c1 = s.conservation()
+    cm.update()
+    _abi.call("sph_neighbor_counts", torch.cuda.current_stream().cuda_stream, cm.grid,
+              pa.view(), pa.view(), 2.0, cnt.data_ptr())
+    pairs1 = int(cnt.sum().item())
+    stages = 2 if args.c1_integrator == "rk2" else 1
+    # density and momentum pass over the same pairs in every stage
+    pairs_per_step = stages * 2 * 0.5 * (pairs0 + pairs1)
+    kt = time_kernels(s, n, args.profile_steps)
+    # algorithmic bytes per launch, FP64 2D, one array: density reads
+    # x,y,h,m and writes rho,p,cs (56 B); stage 1 reads x,y,u,v,h,m,rho,p,cs
+    # and writes k1 (a) and X1 (x,y,u,v) (120 B); stage 2 also reads X0 and
+    # k1 and writes X (152 B); the Euler stage writes X (104 B)
+    kb = {"density_fluid": 56.0 * n, "stage1": 120.0 * n, "stage2": 152.0 * n,
+          "stage0": 104.0 * n}
+    per = {k: float(np.mean(v)) for k, v in kt.items()}
+    dom = max(per, key=lambda k: per[k] * len(kt[k]))
+    achieved = kb.get(dom, 0.0) / (per[dom] * 1e-3) / 1e9
+    names = {"density_fluid": "sph::density_kernel<double, double, true>",
+             "stage1": "sph::momentum_kernel<double, double, 1>",
+             "stage2": "sph::momentum_kernel<double, double, 2>",
+             "stage0": "sph::momentum_kernel<double, double, 3>"}
+    out = {
+        "metric": "steps/s", "value": nsteps / wall, "unit": "steps/s",
+        "n_gpus": 1, "steps": nsteps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / nsteps, "gpu_ms_per_step": e0.elapsed_time(e1) / nsteps,
+        "pair_interactions_per_s": pairs_per_step * nsteps / wall,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "2D elliptical drop (C1), FP64, %s, dt = 1e-5, default "
+                               "Particles/Solver objects" % args.c1_integrator.upper(),
+                   "n": n, "h": 1.3 * 0.025, "c0": 1400.0, "alpha": 0.01,
+                   "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": names.get(dom, dom), "pass": dom,
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": kb.get(dom), "avg_launch_ms": per[dom],
+                     "launches_timed": len(kt[dom])},
+        "kernel_ms_events_in_loop": per,
+        "conservation": conservation_report(c0, c1, args.warmup + nsteps),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = c1_cpu_baseline(args.c1_integrator, nsteps)
+    print(json.dumps(out), flush=True)
+
+
+def c1_cpu_baseline(integrator, nsteps):
+    """The same drop stepped by the FP64 oracle, single thread (the
+    reference's structure) and on all host cores."""
+    from oracle import oracle as O
+    from tests.util import to_oracle
+    from pysph_amd.solver import generators as gen
+    res = {}
+    for threads in (1, len(os.sched_getaffinity(0))):
+        ref = to_oracle(gen.elliptical_drop(dx=0.025, precision="double"))
+        sch = O.scheme(2, integrator=integrator, dt=1e-5, co=1400.0, ro=1.0, gamma=7.0,
+                       alpha=0.01, beta=0.0, eta=0.1, cell_size=2.0 * 1.3 * 0.025)
+        O.set_threads(threads)
+        t0 = time.perf_counter()
+        O.scheme_step(sch, [ref], nsteps)
+        t = time.perf_counter() - t0
+        res[threads] = {"value": nsteps / t, "unit": "steps/s", "cores": threads,
+                        "kind": "port",
+                        "sample": "%d %s steps of the same 5025-particle drop with the FP64 "
+                                  "oracle, %d thread(s), %.2f s"
+                                  % (nsteps, integrator.upper(), threads, t)}
+    O.set_threads(1)
+    base = res[1]
+    base["all_cores"] = res[max(res)]
+    return base
+
+
+def conservation_report(c0, c1, nsteps):
+    """FP64 sums (mass, momentum, kinetic, potential, Tait elastic, total
+    energy) at the start and the end of a run, and the drift of the total
+    energy relative to its start value, per 100 steps."""
+    e0, e1 = float(c0[7]), float(c1[7])
+    return {"mass": float(c1[0]), "mass_rel_change": float((c1[0] - c0[0]) / c0[0]),
+            "momentum": [float(v) for v in c1[1:4]],
+            "kinetic": float(c1[4]), "potential": float(c1[5]), "elastic": float(c1[6]),
+            "total_start": e0, "total_end": e1, "steps": int(nsteps),
+            "energy_drift_per_100_steps": (e1 - e0) / abs(e0) * 100.0 / max(nsteps, 1)
+            if e0 != 0 else None}
+
+
 SURROGATE_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "pysph_amd",
                                  "surrogate", "weights", "dam2d_edge_mlp.npz")
 
@@ -717,6 +847,8 @@ def main():
         return main_batch(args)
     if args.workload == "surrogate2d":
         return main_surrogate(args)
+    if args.workload == "c1":
+        return main_c1(args)
     if args.workload != "dam3d":
         return main_cube(args)
     from pysph_amd.sph import _ops
@@ -730,6 +862,7 @@ def main():
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
+    cons0 = s.conservation()
     for _ in range(args.warmup):
         s.integrator.integrate(s.dt)
     torch.cuda.synchronize()
@@ -745,6 +878,7 @@ def main():
     gpu_ms = e0.elapsed_time(e1)
     t = wall
     particles.cell_manager.verify_jumps()
+    cons1 = s.conservation()
     p_rho1, p_m1, p_rho_fluid1 = pair_counts(particles, f, b)
     p_rho, p_m = 0.5 * (p_rho0 + p_rho1), 0.5 * (p_m0 + p_m1)
     pairs_per_step = 2 * (p_rho + p_m)
@@ -816,6 +950,7 @@ def main():
         # own cost to the launch it brackets)
         "kernel_ms_events_in_loop": {k: float(np.mean(v)) for k, v in kt.items()},
         "pass_ms_per_step": per_step,
+        "conservation": conservation_report(cons0, cons1, args.warmup + args.steps),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(info)

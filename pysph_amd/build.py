@@ -3,6 +3,7 @@
 The library is written in-tree next to this file so that it travels with the
 source tree.  `python -m pysph_amd.build` (or __graft_entry__.build()) runs it.
 """
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -31,18 +32,24 @@ FLAGS = [
 def build(verbose=False, out=LIB, defines=()):
     """Compile every source and link `out`; `defines` are extra -D macros
     (tuning constants) for variant builds."""
-    objs = []
     objdir = os.path.join(HERE, "_obj", os.path.basename(out).replace(".so", ""))
     os.makedirs(objdir, exist_ok=True)
     extra = ["-D" + d for d in defines]
+    cmds = []
+    objs = []
     for src in SOURCES:
         obj = os.path.join(objdir, src.replace(".hip", ".o"))
-        cmd = ([HIPCC] + [f for f in FLAGS if f != "-shared"] + extra +
-               ["-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose:
-            print(" ".join(cmd), flush=True)
-        subprocess.check_call(cmd)
+        cmds.append([HIPCC] + [f for f in FLAGS if f != "-shared"] + extra +
+                    ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    # the sources compile independently: one hipcc per source, in parallel
+    jobs = max(1, min(len(cmds), int(os.environ.get("MAX_JOBS", os.cpu_count() or 1))))
+    with concurrent.futures.ThreadPoolExecutor(jobs) as ex:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd), flush=True)
+        for f in [ex.submit(subprocess.check_call, cmd) for cmd in cmds]:
+            f.result()
     tmp = out + ".tmp"
     cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs
     subprocess.check_call(cmd)

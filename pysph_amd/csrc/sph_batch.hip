@@ -19,6 +19,8 @@
 #include <algorithm>
 #include <vector>
 
+#include <string.h>
+
 #include "sph_fast.h"
 
 namespace sph {
@@ -44,6 +46,13 @@ struct Batch {
   int ntiles[2] = {0, 0};
   int* d_counters = nullptr;  // kXcds per launch site
   float* d_disp = nullptr;    // largest predictor displacement of the step
+  // (x, y, z, m) copies of the fluid and boundary sets (written with pos4 /
+  // vel4) and, per scene, the h-uniformity pair of each set ([scene][set]
+  // -> 2 words: equal FP32 bits = uniform h), so that uniform-h scenes take
+  // the same density and momentum pair paths as a single array does
+  float* d_fpm4 = nullptr;
+  float* d_bpm4 = nullptr;
+  int* d_hrange = nullptr;
 };
 
 constexpr int kSites = 6;
@@ -126,16 +135,19 @@ __global__ __launch_bounds__(kBlock) void copy3_kernel(
   b2[i] = a2[i];
 }
 
-// pos4 = (x, y, z, h), vel4 = (u, v, w, m) from separate component arrays.
+// pos4 = (x, y, z, h), vel4 = (u, v, w, m), pm4 = (x, y, z, m) from separate
+// component arrays.
 __global__ __launch_bounds__(kBlock) void batch_pack_kernel(
     int64_t n, const float* __restrict__ x, const float* __restrict__ y,
     const float* __restrict__ z, const float* __restrict__ h, const float* __restrict__ u,
     const float* __restrict__ v, const float* __restrict__ w, const float* __restrict__ m,
-    float4* __restrict__ pos4, float4* __restrict__ vel4) {
+    float4* __restrict__ pos4, float4* __restrict__ vel4, float4* __restrict__ pm4) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  pos4[i] = make_float4(x[i], y[i], z[i], h[i]);
-  vel4[i] = make_float4(u[i], v[i], w[i], m[i]);
+  const float xi = x[i], yi = y[i], zi = z[i], mi = m[i];
+  pos4[i] = make_float4(xi, yi, zi, h[i]);
+  vel4[i] = make_float4(u[i], v[i], w[i], mi);
+  pm4[i] = make_float4(xi, yi, zi, mi);
 }
 
 // ---------------------------------------------------------------------------
@@ -147,7 +159,8 @@ static int64_t scene_cells(const sph_batch_scene& s) {
 
 static View<float> set_view(const sph_batch_set& S, int64_t off, int64_t n, const float* pos4,
                             const float* vel4, float* aux4, const int32_t* cell,
-                            const int32_t* cstart) {
+                            const int32_t* cstart, const float* pm4 = nullptr,
+                            const int* hrange = nullptr) {
   sph_view v{};
   v.n = n;
   v.dtype = SPH_F32;
@@ -161,6 +174,8 @@ static View<float> set_view(const sph_batch_set& S, int64_t off, int64_t n, cons
   v.pos4 = pos4 ? (void*)(pos4 + 4 * off) : nullptr;
   v.vel4 = vel4 ? (void*)(vel4 + 4 * off) : nullptr;
   v.aux4 = aux4 ? (void*)(aux4 + 4 * off) : nullptr;
+  v.pm4 = pm4 ? (void*)(pm4 + 4 * off) : nullptr;
+  v.hrange = hrange;
   return to_view<float>(v);
 }
 
@@ -198,6 +213,9 @@ static void free_batch(Batch* b) {
     }
   (void)hipFree(b->d_counters);
   (void)hipFree(b->d_disp);
+  (void)hipFree(b->d_fpm4);
+  (void)hipFree(b->d_bpm4);
+  (void)hipFree(b->d_hrange);
   delete b;
 }
 
@@ -364,6 +382,49 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
     free_batch(b);
     return SPH_EINVAL;
   }
+  {
+    // h never changes during a run: the h-uniformity pairs are fixed here
+    std::vector<float> hf((size_t)B.nf_total), hb((size_t)B.nb_total);
+    hipError_t e = hipSuccess;
+    if (B.nf_total > 0)
+      e = hipMemcpy(hf.data(), B.fa.h, hf.size() * 4, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && B.nb_total > 0)
+      e = hipMemcpy(hb.data(), B.bnd_in.h, hb.size() * 4, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+      set_error("sph_batch_create: %s", hipGetErrorString(e));
+      free_batch(b);
+      return (int32_t)e;
+    }
+    std::vector<int> hr((size_t)nscenes * 4, 0);
+    auto pair = [](const float* h, int64_t n, int* out) {
+      if (n <= 0) return;
+      int b0, b1;
+      memcpy(&b0, &h[0], 4);
+      b1 = b0;
+      for (int64_t i = 1; i < n; ++i) {
+        int bi;
+        memcpy(&bi, &h[i], 4);
+        if (bi != b0) {
+          b1 = bi;
+          break;
+        }
+      }
+      out[0] = b0;
+      out[1] = b1;
+    };
+    for (int s = 0; s < nscenes; ++s) {
+      pair(hf.data() + sc[s].f0, sc[s].nf, &hr[4 * s]);
+      pair(hb.data() + sc[s].b0, sc[s].nb, &hr[4 * s + 2]);
+    }
+    if (upload(&b->d_hrange, hr) != SPH_OK ||
+        upload(&b->d_fpm4, std::vector<float>((size_t)std::max<int64_t>(B.nf_total, 1) * 4)) !=
+            SPH_OK ||
+        upload(&b->d_bpm4, std::vector<float>((size_t)std::max<int64_t>(B.nb_total, 1) * 4)) !=
+            SPH_OK) {
+      free_batch(b);
+      return SPH_EINVAL;
+    }
+  }
   std::vector<SceneArgs> args(nscenes);
   std::vector<BatchBinGrid> bgrid(nscenes);
   int64_t base = 0;
@@ -389,9 +450,9 @@ int32_t sph_batch_create(int32_t nscenes, const sph_batch_scene* scenes,
     const sph_batch_set* fset[2] = {&B.fb, &B.fb};
     for (int ph = 0; ph < 2; ++ph) {
       const View<float> fsrc = set_view(*fset[ph], 0, B.nf_total, B.fpos4, B.fvel4, B.faux4,
-                                        B.fcell, fcs);
+                                        B.fcell, fcs, b->d_fpm4, b->d_hrange + 4 * s);
       const View<float> bsrc = set_view(B.bnd, 0, B.nb_total, B.bpos4, B.bvel4, B.baux4,
-                                        B.bcell, bcs);
+                                        B.bcell, bcs, b->d_bpm4, b->d_hrange + 4 * s + 2);
       for (int set = 0; set < 2; ++set) {
         PassArgs<float>& a = A.dens[ph][set];
         a = PassArgs<float>{};
@@ -545,8 +606,9 @@ int32_t sph_batch_pack(void* handle, void* stream, int32_t which) {
   }
   float* pos4 = which == 2 ? B.bpos4 : B.fpos4;
   float* vel4 = which == 2 ? B.bvel4 : B.fvel4;
+  float* pm4 = which == 2 ? b->d_bpm4 : b->d_fpm4;
   hipLaunchKernelGGL(batch_pack_kernel, dim3(nblocks_for(n)), dim3(kBlock), 0, st, n, x, y, z,
-                     S.h, u, v, w, S.m, (float4*)pos4, (float4*)vel4);
+                     S.h, u, v, w, S.m, (float4*)pos4, (float4*)vel4, (float4*)pm4);
   return launch_status("sph_batch_pack");
 }
 

@@ -53,6 +53,16 @@ constexpr int kUnroll = SPH_UNROLL;
 #define SPH_DENS_UNROLL SPH_UNROLL
 #endif
 constexpr int kDensUnroll = SPH_DENS_UNROLL;  // candidates per screen step, density
+// density sources of uniform h are summed without the LDS queue
+#ifndef SPH_DENS_DIRECT
+#define SPH_DENS_DIRECT 1
+#endif
+constexpr bool kDensDirect = SPH_DENS_DIRECT != 0;
+// momentum pairs of a source of uniform h use per-destination constants
+#ifndef SPH_MOM_UNI
+#define SPH_MOM_UNI 1
+#endif
+constexpr bool kMomUni = SPH_MOM_UNI != 0;
 __device__ constexpr float4 kFar4 = {3.0e38f, 3.0e38f, 3.0e38f, 0.f};
 
 // float4 gather at a non-negative index as a 32-bit byte offset from the
@@ -292,6 +302,59 @@ __device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buff
   }
 }
 
+// Summation density over the candidates j0 .. j0+U-1 (those below j1) of
+// one stencil row of a source whose h is uniform: no queue.  Every lane
+// loads U (x, y, z, m) records (pm4, through the bounds-checked buffer view
+// `rs`, one base offset and the slots as immediate offsets), screens them
+// like screen_block (FP32 test, FP64 inside the +-2e-6 band) and adds
+// m W(q) of every accepted one, q = |r_a - r_b| * inv from the screened
+// squared distance, in row order (the order a queue would drain them in).
+// Evaluating W for every candidate costs about what queueing the accepted
+// ones costs (about 11 VALU per candidate either way), and it removes the
+// drain loop with its dependent index read, gather and memory wait per
+// pair.  FP32 pair arithmetic only.
+template <int U>
+__device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int j0, int j1,
+                                                     const float4& pd, float r2lo, float r2hi,
+                                                     double r2d, float inv, float fac,
+                                                     float& nr) {
+  float4 c[U];
+  int off = (int)((unsigned)j0 * 16u);
+  asm("" : "+v"(off));
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * u, 0, 0);
+    c[u] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                       __uint_as_float(v[3]));
+  }
+  const int rem = j1 - j0;
+  bool ok[U], bnd[U];
+  float d2[U];
+  bool band = false;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
+    d2[u] = dx * dx + dy * dy + dz * dz;
+    const bool in = u < rem;
+    ok[u] = in && d2[u] < r2lo;
+    bnd[u] = in && !(d2[u] < r2lo) && !(d2[u] > r2hi);
+    band = band || bnd[u];
+  }
+  if (__any(band)) {
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (bnd[u])
+        ok[u] = exact_within((double)c[u].x, (double)c[u].y, (double)c[u].z, (double)pd.x,
+                             (double)pd.y, (double)pd.z, r2d);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const float q = __builtin_amdgcn_sqrtf(d2[u]) * inv;
+    const float t = c[u].w * (w_sel<float>(q) * fac);
+    nr += ok[u] ? t : 0.f;
+  }
+}
+
 // Whether a list built with the given skin still holds every neighbour of a
 // destination of search radius r after a displacement of at most disp.
 __device__ __forceinline__ bool list_valid(const float* disp, float skin, double r) {
@@ -395,6 +458,34 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
     const bool uni_h = exact_list && S.pm4 != nullptr && S.hrange != nullptr &&
                        S.hrange[0] == S.hrange[1];
     const CT h_src = uni_h ? (CT)__int_as_float(S.hrange[0]) : CT(0);
+    if (kDensDirect && !BUILD && uni_h && sizeof(CT) == 4) {
+      // queue-free summation (density_direct_block): h_ab, 1/h_ab and the
+      // kernel normalisation are fixed per destination and source
+      const float hab = 0.5f * ((float)hd + (float)h_src);
+      const float inv = frcp<float>(hab);
+      const float fac = h_dim_inv<float>(inv, dim);
+      const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+          (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
+      float nrf = 0.f;
+      for (int dz = -ring; dz <= ring; ++dz) {
+        const bool zok = local && dz >= dzlo && dz <= dzhi;
+        if (!__any(zok)) continue;
+        for (int dy = -ring; dy <= ring; ++dy) {
+          const bool yok = zok && dy >= dylo && dy <= dyhi;
+          if (!__any(yok)) continue;
+          int j0 = 0, j1 = 0;
+          if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
+          while (__any(j0 < j1)) {
+            density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv, fac, nrf);
+            j0 += kDensUnroll;
+          }
+        }
+      }
+      nr = (CT)nrf;
+      if (s == a.cache_src && a.cache_mode == 1 && valid) a.cache_sums[i] = (double)nr;
+      rho += nr;
+      continue;
+    }
     auto drain = [&]() {
       if (uni_h) {
         for (int k = 0; __any(k < cnt); ++k) {
@@ -489,6 +580,66 @@ __device__ __forceinline__ void mom_pair_r(const MomDst<CT>& D, const float4& ps
   }
   if (DX) {
     const CT temp = mb * (w_sel<CT>(qq) * fac) * irhoab;
+    ax[0] += temp * -vx;
+    ax[1] += temp * -vy;
+    ax[2] += temp * -vz;
+  }
+}
+
+// Constants of the pairs of one destination with a source whose h is
+// uniform (h_b fixed): h_ab, 1/h_ab, the normalisation 1/h_ab^dim and the
+// gradient factors folded with it.
+struct MomUni {
+  float hab, inv, fac, eta2hh, kout, kin;
+};
+
+__device__ __forceinline__ MomUni mom_uni(float ha, float hb, float eta2, int dim) {
+  MomUni u;
+  u.hab = 0.5f * (ha + hb);
+  u.inv = frcp<float>(u.hab);
+  u.fac = h_dim_inv<float>(u.inv, dim);
+  u.eta2hh = eta2 * u.hab * u.hab;
+  u.kout = -0.75f * u.inv * u.fac;
+  u.kin = 3.0f * (u.inv * u.inv) * u.fac;
+  return u;
+}
+
+// mom_pair_r<float, DM, DX> for a source of uniform h: the per-pair
+// reciprocal of h_ab and the normalisation come from MomUni, and one
+// reciprocal square root gives both |r_ab| and the 1/|r_ab| of the outer
+// gradient branch (kernels.pyx:403-412), so a pair costs three
+// quarter-rate transcendental instructions instead of five.
+template <bool DM, bool DX>
+__device__ __forceinline__ void mom_pair_u(const MomDst<float>& D, const MomUni& U,
+                                           const float4& ps, const float4& vs,
+                                           const float4& as, float* am, float* ax) {
+  float rx, ry, rz;
+  const float d2 = pair_d2<float>(D.pd, ps, rx, ry, rz);
+  const float rs = __builtin_amdgcn_rsqf(d2);
+  const float rr = d2 > 0.f ? d2 * rs : 0.f;  // the self pair: r = 0, not 0 * inf
+  const float qq = rr * U.inv;
+  const float mb = vs.w;
+  const float rhob = as.x;
+  const float irhoab = frcp<float>(0.5f * (D.rhoa + rhob));
+  const float vx = D.vd.x - vs.x, vy = D.vd.y - vs.y, vz = D.vd.z - vs.z;
+  const float Ab = as.z;
+  const float cab = 0.5f * (D.ca + as.y);
+  if (DM) {
+    const float dot = vx * rx + vy * ry + vz * rz;
+    const float mu = U.hab * dot * frcp<float>(d2 + U.eta2hh);
+    const float piab = (-D.alpha * cab * mu + D.beta * mu * mu) * irhoab;
+    const float tmp = D.Aa + Ab + (dot < 0.f ? piab : 0.f);
+    const float t = 2.f - qq;
+    const float outer = U.kout * (t * t) * rs;
+    const float inner = U.kin * (0.75f * qq - 1.f);
+    const float gfac = qq > 2.f ? 0.f : (qq >= 1.f ? outer : (qq > 1e-14f ? inner : 0.f));
+    const float gf = -mb * tmp * gfac;
+    am[0] += gf * rx;
+    am[1] += gf * ry;
+    am[2] += gf * rz;
+  }
+  if (DX) {
+    const float temp = mb * (w_sel<float>(qq) * U.fac) * irhoab;
     ax[0] += temp * -vx;
     ax[1] += temp * -vy;
     ax[2] += temp * -vz;
@@ -615,13 +766,32 @@ __device__ __forceinline__ void momentum_scan_body(const Grid& g, const PassArgs
     CT am[3] = {CT(0), CT(0), CT(0)};
     CT ax[3] = {CT(0), CT(0), CT(0)};
     int cnt = 0;
+    // a source of uniform h (FP32 pairs): per-destination pair constants
+    const bool uni = kMomUni && sizeof(CT) == 4 && S.hrange != nullptr &&
+                     S.hrange[0] == S.hrange[1];
     // one drain loop per role set (the roles are uniform over a source)
     auto drain_r = [&](auto dm, auto dx) {
+      constexpr bool M = decltype(dm)::value, X = decltype(dx)::value;
+      if constexpr (sizeof(CT) == 4) {
+        if (uni) {
+          // formed per drain, so that they hold no registers while screening
+          const MomUni U = mom_uni((float)D.ha, __int_as_float(S.hrange[0]), (float)D.eta2,
+                                   D.dim);
+          for (int k = 0; __any(k < cnt); ++k) {
+            if (k < cnt) {
+              const int j = q[k * 64];
+              mom_pair_u<M, X>(reinterpret_cast<const MomDst<float>&>(D), U, ld4(S.pos4, j),
+                               ld4(S.vel4, j), ld4(S.aux4, j), (float*)am, (float*)ax);
+            }
+          }
+          cnt = 0;
+          return;
+        }
+      }
       for (int k = 0; __any(k < cnt); ++k) {
         if (k < cnt) {
           const int j = q[k * 64];
-          mom_pair_r<CT, decltype(dm)::value, decltype(dx)::value>(
-              D, ld4(S.pos4, j), ld4(S.vel4, j), ld4(S.aux4, j), am, ax);
+          mom_pair_r<CT, M, X>(D, ld4(S.pos4, j), ld4(S.vel4, j), ld4(S.aux4, j), am, ax);
         }
       }
       cnt = 0;

@@ -118,78 +118,117 @@ def test_autotune_then_graph_replays_match_eager():
         eng.close()
 
 
-def _oracle_scene(f, b, info, nsteps):
-    """The FP64 oracle's RK2 steps (integrator.py:581-645 per scene) from the
-    scene's stored FP32 bits."""
+def _orc_scheme(info):
     from oracle import oracle as O
-    from tests.util import to_oracle
-    arrs = [to_oracle(f, 0)] + ([to_oracle(b, 1)] if b is not None else [])
-    sch = O.scheme(2, integrator="rk2", dt=info["dt"], co=info["co"], ro=info["rho0"],
-                   gamma=7.0, alpha=info["alpha"], beta=info["beta"], eta=info["eta"],
-                   gravity=info["g"], xsph_eps=info["eps"] or None, cell_size=2.0 * info["h"])
-    O.scheme_step(sch, arrs, nsteps)
-    return arrs
+    return O.scheme(2, integrator="rk2", dt=info["dt"], co=info["co"], ro=info["rho0"],
+                    gamma=7.0, alpha=info["alpha"], beta=info["beta"], eta=info["eta"],
+                    gravity=info["g"], xsph_eps=info["eps"] or None, cell_size=2.0 * info["h"])
 
 
-def _check_scene_vs_oracle(batch, k, f, b, info, nsteps):
+def _stage_state(eng, k):
+    """X1 and k1 of scene k after a step (the engine's stage buffers, in the
+    step's binning order fb), in the scene's user order."""
+    a, b = int(eng.f_off[k]), int(eng.f_off[k + 1])
+    uid = eng.fb["uid"][a:b].cpu().numpy()
+
+    def user(t):
+        v = t[a:b].cpu().numpy().astype(np.float64)
+        r = np.empty_like(v)
+        r[uid] = v
+        return r
+    return [user(t) for t in eng.x1a], [user(t) for t in eng.k1a]
+
+
+def _step_vs_oracle(eng, k, f, b, info):
+    """One RK2 step of scene k (the engine has just taken it) against the
+    FP64 oracle on the same stored inputs, stage by stage: k1 = (a, dv) at
+    the step's start state X, the density at the engine's predictor X1, and
+    the final velocity u0 + dt/2 (k1 + k2) + dt g with k2 the oracle's at X1
+    (integrator.py:581-645, pressure_funcs.pyx:154-245, xsph_funcs.pyx:31-88,
+    density_funcs.pyx:38-84).  Returns the errors."""
+    from oracle import oracle as O
     from tests.util import max_rel, rel_norm
-    ref = _oracle_scene(f, b, info, nsteps)
-    got = batch.scene_state(k, FIELDS)
-    scale = max(1.0, float(np.max(np.abs(ref[0]["x"]))), float(np.max(np.abs(ref[0]["y"]))))
-    for c in ("x", "y"):
-        # FP32 storage of the coordinates: a few units in the last place
-        assert np.max(np.abs(got[c] - ref[0][c])) < 5e-7 * scale, (k, info["kind"], c)
-    ev = rel_norm(np.stack([got["u"], got["v"]]), np.stack([ref[0]["u"], ref[0]["v"]]))
-    er = max_rel(got["rho"], ref[0]["rho"])
-    assert ev < 1e-5 and er < 1e-5, (k, info["kind"], ev, er)
-    if b is not None:
-        assert max_rel(batch.boundary_state(k, ("rho",))["rho"], ref[1]["rho"]) < 1e-5
-    return ev, er
+    x0 = {c: f[c] for c in ("x", "y", "u", "v")}
+    sch = _orc_scheme(info)
+    rf = O.make_array(ptype=0, **{c: f[c] for c in f})
+    arrs = [rf] + ([O.make_array(ptype=1, **{c: b[c] for c in b})] if b is not None else [])
+    a_o, d_o = O.scheme_rhs(sch, arrs)
+    X1, K1 = _stage_state(eng, k)
+    ek1 = rel_norm(np.stack([K1[0], K1[1]]), a_o[:2])
+    ed1 = rel_norm(np.stack([K1[3], K1[4]]), d_o[:2]) if info["eps"] else 0.0
+    r1 = dict(rf)
+    for c, nm in ((0, "x"), (1, "y"), (3, "u"), (4, "v")):
+        r1[nm] = X1[c]
+    arrs1 = [O.make_array(ptype=0, **{c: r1[c] for c in f})] + arrs[1:]
+    a_o2, d_o2 = O.scheme_rhs(sch, arrs1)
+    got = eng.scene_state(k, ("x", "y", "u", "v", "rho"))
+    erho = max_rel(got["rho"], arrs1[0]["rho"])
+    hdt = 0.5 * info["dt"]
+    g = info["g"] or (0.0, 0.0, 0.0)
+    u_o = [x0[c] + hdt * (K1[i] + a_o2[i]) + hdt * (g[i] + g[i])
+           for i, c in ((0, "u"), (1, "v"))]
+    eu = rel_norm(np.stack([got["u"], got["v"]]), np.stack(u_o))
+    return ek1, ed1, erho, eu
+
+
+def _snapshot(eng, k, with_boundary):
+    st = eng.scene_state(k, ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs"))
+    f = {c: v.astype(np.float64) for c, v in st.items()}
+    b = None
+    if with_boundary:
+        bs = eng.boundary_state(k, ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs"))
+        b = {c: v.astype(np.float64) for c, v in bs.items()}
+    return f, b
+
+
+def _check_steps(eng, ks, sc, nsteps, label):
+    worst = 0.0
+    for step in range(nsteps):
+        snaps = {k: _snapshot(eng, k, sc[k][1] is not None) for k in ks}
+        eng.step()
+        torch.cuda.synchronize()
+        for k in ks:
+            f, b = snaps[k]
+            info = sc[k][2]
+            errs = _step_vs_oracle(eng, k, f, b, info)
+            print("%s step %d scene %d (%s, %d fluid): k1 %.2e, xsph %.2e, rho(X1) %.2e, "
+                  "u %.2e" % ((label, step, k, info["kind"], len(f["x"])) + errs))
+            assert max(errs) < 1e-5, (step, k, info["kind"], errs)
+            worst = max(worst, max(errs))
+    return worst
 
 
 def test_batch_scenes_vs_oracle():
     """Scenes of one batch (elliptical drops and dam breaks of 1k-16k
-    particles) against the FP64 oracle, 3 RK2 steps from the same stored
-    bits: density max relative and velocity normwise within 1e-5."""
-    nsteps = 3
+    particles), 3 RK2 steps; every step checked against the FP64 oracle on
+    the engine's own stored inputs: k1, the density at X1 and the final
+    velocity within 1e-5 (density max relative, the rest normwise)."""
     sc = _scenes(8, seed=11, n_min=1000, n_max=16000)
     assert {i["kind"] for _, _, i in sc} == {"drop", "dam"}
-    batch = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
-    batch.run(nsteps)
-    torch.cuda.synchronize()
-    for k, (f, b, info) in enumerate(sc):
-        ev, er = _check_scene_vs_oracle(batch, k, f, b, info, nsteps)
-        print("scene %d (%s, %d fluid): velocity %.2e, rho %.2e"
-              % (k, info["kind"], f.get_number_of_particles(), ev, er))
+    eng = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
+    _check_steps(eng, list(range(len(sc))), sc, 3, "batch")
 
 
 def test_full_size_batch_vs_oracle():
     """The benchmarked batch itself (512 scenes, 1k-64k fluid particles
     each, one HIP graph per step, autotuned schedules), 3 RK2 steps; the
-    largest and the smallest scene of each kind against the FP64 oracle."""
+    largest and the smallest scene of each kind checked every step against
+    the FP64 oracle on the engine's stored inputs, as above."""
     from oracle import oracle as O
-    nsteps = 3
     sc = gen.batch_scenes(512, seed=0)
-    stream = torch.cuda.Stream()
-    with torch.cuda.stream(stream):
-        eng = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
-        eng.autotune(reps=1)
-        eng.capture()
-        for _ in range(nsteps):
-            eng.step()
-        torch.cuda.synchronize()
     picks = []
     for kind in ("drop", "dam"):
         ks = [k for k, (_, _, i) in enumerate(sc) if i["kind"] == kind]
         ks.sort(key=lambda k: sc[k][0].get_number_of_particles())
         picks += [ks[0], ks[-1]]
+    stream = torch.cuda.Stream()
     O.set_threads(0)
     try:
-        for k in picks:
-            f, b, info = sc[k]
-            ev, er = _check_scene_vs_oracle(eng, k, f, b, info, nsteps)
-            print("scene %d (%s, %d fluid): velocity %.2e, rho %.2e"
-                  % (k, info["kind"], f.get_number_of_particles(), ev, er))
+        with torch.cuda.stream(stream):
+            eng = BatchedWCSPH([scene_from_arrays(f, b, i) for f, b, i in sc])
+            eng.autotune(reps=1)
+            eng.capture()
+            _check_steps(eng, picks, sc, 3, "full batch")
+            eng.close()
     finally:
         O.set_threads(1)
-        eng.close()
