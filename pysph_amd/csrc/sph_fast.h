@@ -38,14 +38,21 @@
 
 namespace sph {
 
-// LDS queue depth per lane and candidate loads in flight per lane
+// LDS queue depth per lane and candidate loads in flight per lane; the
+// momentum scans' 24 entries keep six 256-lane blocks within the 160 KiB of
+// LDS, the occupancy their 80 VGPRs allow
 #ifndef SPH_QUEUE
-#define SPH_QUEUE 20
+#define SPH_QUEUE 24
+#endif
+// LDS queue depth of the density scans (sources of varied h, list builds)
+#ifndef SPH_DENS_QUEUE
+#define SPH_DENS_QUEUE 16
 #endif
 #ifndef SPH_UNROLL
 #define SPH_UNROLL 4
 #endif
 constexpr int kQueue = SPH_QUEUE;
+constexpr int kDensQueue = SPH_DENS_QUEUE;
 // occupancy target of the momentum scan kernels (waves per SIMD)
 constexpr int kMomWaves = 6;
 constexpr int kUnroll = SPH_UNROLL;
@@ -58,6 +65,14 @@ constexpr int kDensUnroll = SPH_DENS_UNROLL;  // candidates per screen step, den
 #define SPH_DENS_DIRECT 1
 #endif
 constexpr bool kDensDirect = SPH_DENS_DIRECT != 0;
+// a density pass over two sources of uniform h sums them row by row
+// together (one source, or more than two, take the source-by-source loop)
+constexpr int kDirectSrc = 2;
+// occupancy target of the density scan kernels (waves per SIMD, 0: none):
+// seven waves hold them within 72 VGPRs without spilling
+#ifndef SPH_DENS_WAVES
+#define SPH_DENS_WAVES 7
+#endif
 // momentum pairs of a source of uniform h use per-destination constants
 #ifndef SPH_MOM_UNI
 #define SPH_MOM_UNI 1
@@ -186,6 +201,29 @@ __device__ __forceinline__ void axis_span(float v, float r, int c, int n, int lo
 
 // Particle range of the cells of grid row (y, z) within the stencil ring of
 // cell column cx whose extent can hold a point within the cull radius.
+// The cell-start indices [k0, k1] (read cstart[k0] and cstart[k1]) of that
+// range; false when the row holds no candidate cell.
+__device__ __forceinline__ bool row_cells_cull(const Grid& g, int cx, int y, int z, int ring,
+                                               const Cull& cu, int& k0, int& k1) {
+  if (y < 0 || y >= g.dims[1] || z < 0 || z >= g.dims[2]) return false;
+  const float gy = axis_gap(cu.yd, y, g.dims[1], g.lo[1], g.cs_f);
+  const float gz = axis_gap(cu.zd, z, g.dims[2], g.lo[2], g.cs_f);
+  const float w2 = cu.rc2 - gy * gy - gz * gz;
+  if (!(w2 >= 0.f)) return false;
+  const float w = __builtin_amdgcn_sqrtf(w2);
+  const float last = (float)(g.dims[0] - 1);
+  const float fa = fminf(fmaxf(floorf((cu.xd - w) * g.inv_cs_f) - (float)g.lo[0], 0.f), last);
+  const float fb = fminf(fmaxf(floorf((cu.xd + w) * g.inv_cs_f) - (float)g.lo[0], 0.f), last);
+  int x0 = max(cx - ring, 0), x1 = min(cx + ring, g.dims[0] - 1);
+  if (fa > (float)x1 || fb < (float)x0) return false;  // NaN positions keep the ring
+  if (fa > (float)x0) x0 = (int)fa;
+  if (fb < (float)x1) x1 = (int)fb;
+  const int base = (z * g.dims[1] + y) * g.dims[0];
+  k0 = base + x0;
+  k1 = base + x1 + 1;
+  return true;
+}
+
 __device__ __forceinline__ void row_range_cull(const Grid& g, const int* __restrict__ cstart,
                                                int cx, int y, int z, int ring, const Cull& cu,
                                                int& j0, int& j1) {
@@ -409,7 +447,7 @@ __device__ __forceinline__ void density_epilogue(const PassArgs<float>& a, int i
 }
 
 // Cell scan of destination i (all lanes of the wave call it together; q
-// is the lane's LDS queue, kQueue ints 64 apart); BUILD also records the
+// is the lane's LDS queue, kDensQueue ints 64 apart); BUILD also records the
 // list (superset radius).
 // DIM > 0 fixes the dimension at compile time (the kernels of dimensions 2
 // and 3); DIM = 0 reads it from the arguments.
@@ -441,6 +479,73 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
   const int64_t ndst = a.dst.n;
   CT rho = CT(0);
   int lpos = 0;
+  if constexpr (kDensDirect && !BUILD && sizeof(CT) == 4) {
+    // every source of uniform h: the stencil rows outside, the sources
+    // inside, so that a row's culled cell range is computed once for all
+    // sources; each source keeps its own sum, added in source order (the
+    // same bits as summing source by source)
+    bool all_uni = a.nsrc >= 2 && a.nsrc <= kDirectSrc;
+    for (int s = 0; s < a.nsrc; ++s) {
+      const View<float> S = a.src[s];
+      all_uni = all_uni && S.pm4 != nullptr && S.hrange != nullptr &&
+                S.hrange[0] == S.hrange[1];
+    }
+    if (all_uni) {
+      float inv[kDirectSrc], fac[kDirectSrc], nrs[kDirectSrc];
+      bool scan[kDirectSrc];
+#pragma unroll
+      for (int s = 0; s < kDirectSrc; ++s) {
+        nrs[s] = 0.f;
+        scan[s] = s < a.nsrc && !(s == a.cache_src && a.cache_mode == 2);
+        const float hb = s < a.nsrc ? __int_as_float(a.src[s].hrange[0]) : 1.f;
+        const float hab = 0.5f * ((float)hd + hb);
+        inv[s] = frcp<float>(hab);
+        fac[s] = h_dim_inv<float>(inv[s], dim);
+      }
+      for (int dz = -ring; dz <= ring; ++dz) {
+        const bool zok = local && dz >= dzlo && dz <= dzhi;
+        if (!__any(zok)) continue;
+        for (int dy = -ring; dy <= ring; ++dy) {
+          const bool yok = zok && dy >= dylo && dy <= dyhi;
+          if (!__any(yok)) continue;
+          int k0 = 0, k1 = 0;
+          const bool row = yok && row_cells_cull(g, cx, cy + dy, cz + dz, ring, cu, k0, k1);
+#pragma unroll
+          for (int s = 0; s < kDirectSrc; ++s) {
+            if (!scan[s]) continue;
+            const View<float> S = a.src[s];
+            int j0 = 0, j1 = 0;
+            if (row) {
+              j0 = S.cstart[k0];
+              j1 = S.cstart[k1];
+            }
+            if (!__any(j0 < j1)) continue;
+            const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+                (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
+            do {
+              density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv[s], fac[s],
+                                                nrs[s]);
+              j0 += kDensUnroll;
+            } while (__any(j0 < j1));
+          }
+        }
+      }
+#pragma unroll
+      for (int s = 0; s < kDirectSrc; ++s) {
+        if (s >= a.nsrc) continue;
+        CT nr = (CT)nrs[s];
+        if (s == a.cache_src && a.cache_mode == 2) {
+          if (local) nr = (CT)a.cache_sums[i];
+        } else if (s == a.cache_src && a.cache_mode == 1 && valid) {
+          a.cache_sums[i] = (double)nr;
+        }
+        rho += nr;
+      }
+      if (!valid) return;
+      density_epilogue<TAIT>(a, i, local, (double)(rho * (CT)a.fac1));
+      return;
+    }
+  }
   for (int s = 0; s < a.nsrc; ++s) {
     const View<float> S = a.src[s];
     const __amdgpu_buffer_rsrc_t rs = cand_rsrc(S);
@@ -520,7 +625,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
         int j0 = 0, j1 = 0;
         if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
         while (__any(j0 < j1)) {
-          if (__any(cnt > kQueue - kDensUnroll)) drain();
+          if (__any(cnt > kDensQueue - kDensUnroll)) drain();
           screen_block<kDensUnroll>(S, rs, j0, j1, pd, r2lo, r2hi, r2d, !exact_list, rl2f, q,
                                     cnt);
           j0 += kDensUnroll;

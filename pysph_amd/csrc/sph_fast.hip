@@ -6,8 +6,12 @@ namespace sph {
 
 
 template <typename CT, bool TAIT, bool BUILD, int DIM>
-__global__ __launch_bounds__(kBlock) void density_scan(Grid g, PassArgs<float> a) {
-  __shared__ int qbuf[kBlock / 64][kQueue][64];
+__global__ __launch_bounds__(kBlock)
+#if SPH_DENS_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
+#endif
+void density_scan(Grid g, PassArgs<float> a) {
+  __shared__ int qbuf[kBlock / 64][kDensQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);
   density_scan_body<CT, TAIT, BUILD, DIM>(g, a, bid * kBlock + threadIdx.x,
