@@ -374,8 +374,9 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
     const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
     d2[u] = dx * dx + dy * dy + dz * dz;
     const bool in = u < rem;
-    ok[u] = in && d2[u] < r2lo;
-    bnd[u] = in && !(d2[u] < r2lo) && !(d2[u] > r2hi);
+    const bool lo = d2[u] < r2lo;
+    ok[u] = in && lo;
+    bnd[u] = in && !lo && d2[u] <= r2hi;
     band = band || bnd[u];
   }
   if (__any(band)) {
@@ -388,8 +389,13 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const float q = __builtin_amdgcn_sqrtf(d2[u]) * inv;
-    const float t = c[u].w * (w_sel<float>(q) * fac);
-    nr += ok[u] ? t : 0.f;
+    // W(q) of kernels.pyx:367-374 without a branch: t = max(2 - q, 0) makes
+    // the outer branch exactly 0 from q = 2 on
+    const float t = fmaxf(2.f - q, 0.f);
+    const float outer = 0.25f * t * t * t;
+    const float inner = 1.f - 1.5f * (q * q) * (1.f - 0.5f * q);
+    const float w = q > 1.f ? outer : inner;
+    nr += ok[u] ? c[u].w * (w * fac) : 0.f;
   }
 }
 
@@ -569,9 +575,9 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
       const float hab = 0.5f * ((float)hd + (float)h_src);
       const float inv = frcp<float>(hab);
       const float fac = h_dim_inv<float>(inv, dim);
+      float nrf = 0.f;
       const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
           (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
-      float nrf = 0.f;
       for (int dz = -ring; dz <= ring; ++dz) {
         const bool zok = local && dz >= dzlo && dz <= dzhi;
         if (!__any(zok)) continue;
