@@ -208,6 +208,17 @@ int32_t sph_neighbor_counts(void *stream, const sph_grid *g, const sph_view *dst
  * the total through *total_host. */
 int32_t sph_scan_counts(void *stream, int64_t n, const int64_t *counts_dev,
                         int64_t *offsets_dev, int64_t *total_host);
+/* sph_scan_counts without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The scan's
+ * tile partials live in a fixed device array inside the library (512 KiB per
+ * GPU, held until the process ends), so n is limited to
+ * SPH_SCAN_NOALLOC_MAX_COUNTS (2^27) and larger n returns SPH_EINVAL.  Calls
+ * that overlap on one GPU share that array and race; serialise them (one
+ * stream, or events).  total_host may be NULL (no synchronisation). */
+#define SPH_SCAN_NOALLOC_MAX_COUNTS (1LL << 27)
+int32_t sph_scan_counts_noalloc(void *stream, int64_t n, const int64_t *counts_dev,
+                                int64_t *offsets_dev, int64_t *total_host);
 /* Fill the CSR rows.  cap < offsets[n] -> SPH_ECAPACITY with *needed_host. */
 int32_t sph_neighbor_fill(void *stream, const sph_grid *g, const sph_view *dst,
                           const sph_view *src, double radius_scale,

@@ -8,6 +8,8 @@
 // (nnps.pyx:271-356, :525-620).
 #include <math.h>
 
+#include <mutex>
+
 #include "sph_internal.h"
 
 namespace sph {
@@ -803,6 +805,56 @@ int32_t sph_varh_neighbor_counts(void* stream, const sph_grid* g, const sph_view
                     "sph_varh_neighbor_counts");
 }
 
+}  // extern "C"
+
+namespace sph {
+
+// Scratch of sph_scan_counts_noalloc: the tile partials of the exclusive
+// scan, in a file-scope device array instead of a per-call allocation.
+// Each GPU holds its own copy until the process ends; host code uses the
+// address hipGetSymbolAddress returns for the current device, never the
+// array's name.  Calls that overlap on one GPU share it and race.
+constexpr int kNoAllocPartials = 1 << 16;  // 512 KiB of int64 partials
+constexpr int64_t kNoAllocMaxCounts = (int64_t)kNoAllocPartials * kScanTile;
+static __device__ __align__(256) int64_t scratch[kNoAllocPartials];
+static_assert(kNoAllocMaxCounts == SPH_SCAN_NOALLOC_MAX_COUNTS,
+              "the header's limit is the scratch's capacity");
+
+constexpr int kNoAllocMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static int64_t* g_scratch_ptr[kNoAllocMaxDevices] = {};
+
+// The current device's address of `scratch`, looked up once per device.
+static int32_t noalloc_scratch(int64_t** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("sph_scan_counts_noalloc: %s", hipGetErrorString(e));
+    return (int32_t)e;
+  }
+  if (dev < 0 || dev >= kNoAllocMaxDevices) {
+    set_error("sph_scan_counts_noalloc: device %d beyond %d", dev, kNoAllocMaxDevices);
+    return SPH_EUNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || p == nullptr) {
+      set_error("sph_scan_counts_noalloc: hipGetSymbolAddress: %s",
+                e != hipSuccess ? hipGetErrorString(e) : "null address");
+      return e != hipSuccess ? (int32_t)e : SPH_EINVAL;
+    }
+    g_scratch_ptr[dev] = (int64_t*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return SPH_OK;
+}
+
+}  // namespace sph
+
+extern "C" {
+
 int32_t sph_scan_counts(void* stream, int64_t n, const int64_t* counts_dev,
                         int64_t* offsets_dev, int64_t* total_host) {
   if (n < 0 || !offsets_dev || (n > 0 && !counts_dev)) {
@@ -825,6 +877,35 @@ int32_t sph_scan_counts(void* stream, int64_t n, const int64_t* counts_dev,
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
       set_error("sph_scan_counts: %s", hipGetErrorString(e));
+      return (int32_t)e;
+    }
+  }
+  return SPH_OK;
+}
+
+int32_t sph_scan_counts_noalloc(void* stream, int64_t n, const int64_t* counts_dev,
+                                int64_t* offsets_dev, int64_t* total_host) {
+  if (n < 0 || !offsets_dev || (n > 0 && !counts_dev)) {
+    set_error("sph_scan_counts_noalloc: bad arguments");
+    return SPH_EINVAL;
+  }
+  if (n > kNoAllocMaxCounts) {
+    set_error("sph_scan_counts_noalloc: %lld counts exceed the maximum of %lld (scratch of "
+              "%d tile partials); use sph_scan_counts",
+              (long long)n, (long long)kNoAllocMaxCounts, kNoAllocPartials);
+    return SPH_EINVAL;
+  }
+  int64_t* partial = nullptr;
+  SPH_TRY(noalloc_scratch(&partial));
+  hipStream_t st = as_stream(stream);
+  // every partial the scan reads is written by the same call first, so the
+  // previous call's contents need no clearing
+  SPH_TRY((exclusive_scan<int64_t, int64_t>(st, counts_dev, n, offsets_dev, partial)));
+  if (total_host) {
+    hipError_t e = hipMemcpyAsync(total_host, offsets_dev + n, 8, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) {
+      set_error("sph_scan_counts_noalloc: %s", hipGetErrorString(e));
       return (int32_t)e;
     }
   }
