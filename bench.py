@@ -47,8 +47,13 @@ def parse():
                     help="shrink the lattice (for quick checks); 1.0 = the named workload")
     ap.add_argument("--pair-prec", choices=["f64", "native"], default="native")
     ap.add_argument("--profile-steps", type=int, default=10)
-    ap.add_argument("--cells-per-radius", type=int, default=2,
-                    help="cell size = 2h / k (neighbour sets are unchanged)")
+    ap.add_argument("--cells-per-radius", type=int, default=None,
+                    help="bins per reference cell of 2h (neighbour sets are unchanged); "
+                         "default 2, with --dropin the solver's default")
+    ap.add_argument("--dropin", action="store_true",
+                    help="dam3d: the solver as a reference script drives it -- grid from "
+                         "the particle bounds, per-binning jump check, the solver's "
+                         "default binning, steps timed through Solver.solve()")
     ap.add_argument("--no-packed", action="store_true",
                     help="use the SoA kernels instead of the packed fast path")
     ap.add_argument("--list", action="store_true",
@@ -65,7 +70,7 @@ def parse():
 
 
 def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, nlist=False,
-               reuse=None, cache_static=False):
+               reuse=None, cache_static=False, dropin=False):
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
     from pysph_amd.solver import generators as gen
@@ -74,9 +79,10 @@ def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, n
     f, b, info = gen.dam_break_3d(dx=1.0 / 135, column=col, tank=tank, seed=0,
                                   precision=precision)
     particles = base.Particles([f, b], update_particles=False)
-    lo, hi = info["domain"]
-    particles.cell_manager.set_fixed_domain(lo, hi)
-    particles.cell_manager.check_jumps = "deferred"
+    if not dropin:
+        lo, hi = info["domain"]
+        particles.cell_manager.set_fixed_domain(lo, hi)
+        particles.cell_manager.check_jumps = "deferred"
     s = solver.wcsph_solver(3, integrator="rk2", co=info["co"], ro=info["rho0"], gamma=7.0,
                             alpha=info["alpha"], beta=info["beta"], eta=info["eta"],
                             gravity=info["g"], xsph_eps=info["eps"])
@@ -84,7 +90,7 @@ def build_case(scale=1.0, precision="single", cells_per_radius=2, packed=True, n
     s.set_time_step(info["dt"])
     plan = s.integrator.plan
     if plan is not None:
-        if cells_per_radius != 1:
+        if cells_per_radius is not None:
             plan.set_cells_per_radius(cells_per_radius)
         plan.use_packed = plan.use_packed and packed
         if nlist or reuse is not None:
@@ -126,17 +132,24 @@ def config_key(args):
 def kernel_names(plan):
     """Kernel symbols behind each pass (as rocprofv3 lists them)."""
     if not plan.use_packed:
-        return {"density_fluid": "sph::density_kernel<float, ...>",
-                "stage1": "sph::momentum_kernel<float, 1>",
-                "stage2": "sph::momentum_kernel<float, 2>"}
-    d = plan.dim
-    if plan.use_list:
-        return {"density_fluid": "sph::density_scan<float, true, true, %d>" % d,
-                "stage1": "sph::momentum_list<float, 1>",
-                "stage2": "sph::momentum_list<float, 2>"}
-    return {"density_fluid": "sph::density_scan<float, true, false, %d>" % d,
-            "stage1": "sph::momentum_scan<float, 1, %d>" % d,
-            "stage2": "sph::momentum_scan<float, 2, %d>" % d}
+        n = {"density_fluid": "sph::density_kernel<float, float, true>",
+             "stage1": "sph::momentum_kernel<float, float, 1>",
+             "stage2": "sph::momentum_kernel<float, float, 2>"}
+    else:
+        d = plan.dim
+        if plan.use_list:
+            n = {"density_fluid": "sph::density_scan<float, true, true, %d>" % d,
+                 "stage1": "sph::momentum_list<float, 1>",
+                 "stage2": "sph::momentum_list<float, 2>"}
+        else:
+            n = {"density_fluid": "sph::density_scan<float, true, false, %d>" % d,
+                 "stage1": "sph::momentum_scan<float, 1, %d>" % d,
+                 "stage2": "sph::momentum_scan<float, 2, %d>" % d}
+    # the boundary's density pass runs the density kernel without lists
+    n["density_boundary"] = n["density_fluid"].replace("true, true", "true, false")
+    # both density passes of a stage in one launch
+    n["density"] = "sph::density_scan2<float, true, %d>" % plan.dim
+    return n
 
 
 def kernel_bytes(nf, nb):
@@ -152,6 +165,8 @@ def kernel_bytes(nf, nb):
         "stage1": 92 * nf + 44 * nb,
         # stage 2: X1 + h,m,rho,p,cs (44) + X0 (24) + k1 (24) read, X (24) written
         "stage2": 116 * nf + 44 * nb,
+        # the fluid's and the boundary's density passes in one launch
+        "density": 52 * nf + 52 * nb,
     }
 
 
@@ -168,6 +183,8 @@ class LaunchTimer:
         self.cur = None
 
     def key(self, name, args):
+        if name == "sph_density_multi":
+            return "density"
         if name in ("sph_density", "sph_density_cached"):
             return "density_fluid" if args[3][0].n == self.nf else "density_boundary"
         if name == "sph_wcsph_stage":
@@ -211,40 +228,72 @@ def time_kernels(s, nf, nsteps):
     return timer.durations()
 
 
-def batch_launch_ms(s, dom, reps=20):
-    """Average launch duration of the dominant pass's kernel: `reps`
-    back-to-back launches between two events on the launch stream, after
-    one full step.  The launches are idempotent: the stage-1 kernel writes
-    X1 and k1 from X, the density pass rewrites rho, p, cs from positions."""
+def _ev_ms(fn, reps):
+    """Average duration (ms) of `reps` back-to-back calls of fn between two
+    HIP events on the current (launch) stream, after one untimed call."""
     import torch
-    from pysph_amd.sph import _ops
-    plan = s.integrator.plan
-    s.integrator.integrate(s.dt)
-    b = plan._buffers()
-    ov = plan._override()
-    if dom == "momentum_stage":
-        key = "stage1"
-
-        def launch():
-            plan._stage(1, s.dt, b, override=ov)
-    else:
-        key = "density_fluid"
-        c = next(c for c in plan.rho_calcs if c.dest is plan.fluid)
-        pa = c.dest
-
-        def launch():
-            _ops.density(plan.cm, plan.prm, pa, c.sources, pa.get_carray("rho"),
-                         pa.get_carray("p"), pa.get_carray("cs"), override=ov)
-    launch()
+    fn()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
     for _ in range(reps):
-        launch()
+        fn()
     e1.record()
     torch.cuda.synchronize()
-    return key, e0.elapsed_time(e1) / reps
+    return e0.elapsed_time(e1) / reps
+
+
+def isolated_launch_ms(s, reps=20):
+    """Average launch duration (ms) of every hot pass, each launched `reps`
+    times back to back on the launch stream after one full step: the
+    density passes of the fluid and of the boundary (idempotent: rho, p, cs
+    from positions), the stage-1 kernel (writes X1 and k1 from X) and the
+    stage-2 kernel reading X1, with its final state written to scratch
+    arrays instead of in place, so that the repeated launches read the same
+    inputs."""
+    import torch
+    from pysph_amd import _abi
+    from pysph_amd._device import ptr
+    from pysph_amd.sph import _ops
+    plan = s.integrator.plan
+    s.integrator.integrate(s.dt)
+    b = plan._buffers()
+    ov = plan._override()
+    out = {}
+    if plan._fuse_density():
+        # the step's launch: both arrays' density passes in one
+        out["density"] = _ev_ms(lambda: plan._density(override=ov), reps)
+    else:
+        for c in plan.rho_calcs:
+            pa = c.dest
+            key = "density_fluid" if pa is plan.fluid else "density_boundary"
+
+            def dens(c=c, pa=pa):
+                _ops.density(plan.cm, plan.prm, pa, c.sources, pa.get_carray("rho"),
+                             pa.get_carray("p"), pa.get_carray("cs"), override=ov)
+            out[key] = _ev_ms(dens, reps)
+    stream = torch.cuda.current_stream().cuda_stream
+    _abi.call("sph_fill_i32", stream, b["disp"].data_ptr(), 1, 0)
+    out["stage1"] = _ev_ms(lambda: plan._stage(1, s.dt, b, override=ov, disp_out=b["disp"]),
+                           reps)
+    s1 = plan._s1_fields(b)
+    plan._pack(plan.fluid, fields=s1, force=True)
+    ov2 = plan._override(s1)
+    pa = plan.fluid
+    scratch = [torch.empty_like(t) for t in plan._state0()]
+    vd = _ops.views([pa], ov2)
+    vs = _ops.views(plan.srcs, ov2)
+    s0 = _abi.ptr_array([ptr(t) for t in plan._state0()])
+    k1 = _abi.ptr_array([ptr(t) for t in b["k1"]])
+    o2 = _abi.ptr_array([ptr(t) for t in scratch])
+    rmax = _ops.search_radius(plan.cm, plan.radius_scale)
+
+    def stage2():
+        _abi.call("sph_wcsph_stage", stream, 2, plan.cm.grid, plan.prm, vd, len(plan.srcs), vs,
+                  plan.flags, rmax, ptr(b["disp"]), float(s.dt), s0, k1, o2, None, None)
+    out["stage2"] = _ev_ms(stage2, reps)
+    return out
 
 
 def source_digest():
@@ -635,9 +684,9 @@ def main_c1(args):
     _abi.call("sph_neighbor_counts", torch.cuda.current_stream().cuda_stream, cm.grid,
               pa.view(), pa.view(), 2.0, cnt.data_ptr())
     pairs0 = int(cnt.sum().item())
-    c0 = s.conservation()
     for _ in range(args.warmup):
         s.integrator.integrate(dt)
+    c0 = s.conservation()
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
@@ -687,7 +736,7 @@ def main_c1(args):
                      "bytes_per_launch": kb.get(dom), "avg_launch_ms": per[dom],
                      "launches_timed": len(kt[dom])},
         "kernel_ms_events_in_loop": per,
-        "conservation": conservation_report(c0, c1, args.warmup + nsteps),
+        "conservation": conservation_report(c0, c1, nsteps),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = c1_cpu_baseline(args.c1_integrator, nsteps)
@@ -830,6 +879,8 @@ def surrogate_cpu_baseline(info):
 
 def main():
     args = parse()
+    if args.cells_per_radius is None and not args.dropin:
+        args.cells_per_radius = 2
     if args.gpus != 1:
         print("bench.py: this build runs on exactly one GPU (the particle step of one "
               "rank's share); --gpus %d is not supported" % args.gpus, file=sys.stderr)
@@ -858,20 +909,31 @@ def main():
                                                      cells_per_radius=args.cells_per_radius,
                                                      packed=not args.no_packed,
                                                      nlist=args.list, reuse=args.reuse,
-                                                     cache_static=args.cache_static)
+                                                     cache_static=args.cache_static,
+                                                     dropin=args.dropin)
+    if args.cells_per_radius is None:
+        args.cells_per_radius = s.integrator.plan.cells_per_radius
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
-    cons0 = s.conservation()
     for _ in range(args.warmup):
         s.integrator.integrate(s.dt)
+    # FP64 sums at the start of the timed steps (after the warm-up, whose
+    # first density evaluation replaces the generator's rho = rho0)
+    cons0 = s.conservation()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for _ in range(args.steps):
-        s.integrator.integrate(s.dt)
+    if args.dropin:
+        s.set_print_freq(0)
+        s.set_final_time(s.t + (args.steps - 0.5) * s.dt)
+        s.solve()
+        assert s.count == args.steps, (s.count, args.steps)
+    else:
+        for _ in range(args.steps):
+            s.integrator.integrate(s.dt)
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -887,22 +949,41 @@ def main():
     kb = kernel_bytes(nf, nb)
     plan = s.integrator.plan
     names = kernel_names(plan)
-    # passes grouped by kernel: the density kernel on fluid destinations
-    # (twice per step) and the momentum/stage kernel (stage 1 + stage 2)
-    fam = {"density_fluid": ["density_fluid"], "momentum_stage": ["stage1", "stage2"]}
-    per_step = {f: sum(sum(kt.get(k, [])) for k in ks) / args.profile_steps
-                for f, ks in fam.items()}
-    dom = max(per_step, key=per_step.get)
-    key, dom_ms = batch_launch_ms(s, dom, reps=20)
-    dom_bytes = float(kb[key])
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    iso = isolated_launch_ms(s, reps=20)
     cfg_key = config_key(args)
     traffic, traffic_src = measured_traffic(cfg_key)
-    dom_traffic = traffic.get(key) if traffic else None
-    # algorithmic flops of the dominant launch (20 per density pair, 80 per
-    # momentum pair)
+    # algorithmic flops: 20 per density pair, 80 per momentum pair
     p_rho_f = 0.5 * (p_rho_fluid0 + p_rho_fluid1)
-    flops_dom = 20.0 * p_rho_f if dom == "density_fluid" else 80.0 * p_m
+    p_rho_b = p_rho - p_rho_f
+    flops = {"density_fluid": 20.0 * p_rho_f, "density_boundary": 20.0 * p_rho_b,
+             "density": 20.0 * p_rho, "stage1": 80.0 * p_m, "stage2": 80.0 * p_m}
+    # passes grouped by the kernel that runs them, with launches per step
+    groups = {}
+    for key, per_step in (("density", 2), ("density_fluid", 2), ("density_boundary", 2),
+                          ("stage1", 1), ("stage2", 1)):
+        if key not in iso:
+            continue
+        groups.setdefault(names[key], []).append((key, per_step))
+    kernels = []
+    for kname, passes in groups.items():
+        launches = sum(n for _, n in passes)
+        ms_step = sum(n * iso[k] for k, n in passes)
+        bytes_step = sum(n * float(kb[k]) for k, n in passes)
+        flops_step = sum(n * flops[k] for k, n in passes)
+        ach = bytes_step / (ms_step * 1e-3) / 1e9
+        tr = None
+        if traffic and all(traffic.get(k) is not None for k, _ in passes):
+            tr = sum(n * traffic[k] for k, n in passes) / launches
+        kernels.append({
+            "kernel": kname, "passes": [k for k, _ in passes], "launches_per_step": launches,
+            "ms_per_step": ms_step, "avg_launch_ms": ms_step / launches,
+            "bytes_per_launch": bytes_step / launches, "achieved": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": tr,
+            "traffic_over_algorithmic": (tr / (bytes_step / launches)) if tr else None,
+            "valu_tflops": flops_step / (ms_step * 1e-3) / 1e12,
+            "valu_frac": flops_step / (ms_step * 1e-3) / 1e12 / FP32_PEAK_TFLOPS})
+    kernels.sort(key=lambda k: -k["ms_per_step"])
+    dom = kernels[0]
     out = {
         "metric": "pair-interactions/s",
         "value": value,
@@ -931,26 +1012,35 @@ def main():
                    # pairs are evaluated once, not in every stage
                    "pairs_evaluated_per_step": pairs_per_step - (
                        2 * pair_counts.boundary_boundary if args.cache_static else 0),
+                   "driver": ("Solver.solve (grid from particle bounds, per-binning jump "
+                              "check)" if args.dropin else
+                              "integrator steps (fixed domain, deferred jump check)"),
                    "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": names[key], "pass": key,
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": dom_traffic,
-                     "traffic_source": traffic_src if traffic else traffic_src,
-                     "bytes_per_launch": dom_bytes, "avg_launch_ms": dom_ms,
-                     "launches_timed": 20},
-        "valu": {"kernel": key, "flops_per_launch": flops_dom,
-                 "achieved_tflops": flops_dom / (dom_ms * 1e-3) / 1e12,
-                 "peak_tflops": FP32_PEAK_TFLOPS,
-                 "frac": flops_dom / (dom_ms * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+        # the kernel with the most GPU time per step, all its launches of a
+        # step: algorithmic bytes per launch over the average isolated launch
+        # duration (HIP events around 20 back-to-back launches of each pass
+        # on the launch stream)
+        "roofline": {"bound": "hbm", "kernel": dom["kernel"], "passes": dom["passes"],
+                     "achieved": dom["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": dom["frac"], "traffic": dom["traffic"],
+                     "traffic_source": traffic_src,
+                     "bytes_per_launch": dom["bytes_per_launch"],
+                     "avg_launch_ms": dom["avg_launch_ms"],
+                     "launches_per_step": dom["launches_per_step"], "launches_timed": 20},
+        "roofline_kernels": kernels,
+        "valu": {"kernel": dom["kernel"], "achieved_tflops": dom["valu_tflops"],
+                 "peak_tflops": FP32_PEAK_TFLOPS, "frac": dom["valu_frac"],
                  # the kernels are built without packed FP32 instructions:
                  # one FP32 FMA per lane and cycle is half the published peak
                  "peak_tflops_unpacked": FP32_PEAK_TFLOPS / 2,
-                 "frac_unpacked": flops_dom / (dom_ms * 1e-3) / 1e12 / (FP32_PEAK_TFLOPS / 2)},
+                 "frac_unpacked": dom["valu_frac"] * 2},
+        "isolated_launch_ms": iso,
         # per-launch events inside the step loop (each event pair adds its
         # own cost to the launch it brackets)
         "kernel_ms_events_in_loop": {k: float(np.mean(v)) for k, v in kt.items()},
-        "pass_ms_per_step": per_step,
-        "conservation": conservation_report(cons0, cons1, args.warmup + args.steps),
+        "pass_ms_per_step_in_loop": {k: float(np.sum(v)) / args.profile_steps
+                                     for k, v in kt.items()},
+        "conservation": conservation_report(cons0, cons1, args.steps),
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(info)

@@ -102,6 +102,12 @@ typedef struct sph_grid {
      * axis.  The packed fast path and neighbour lists are not used then. */
     double period[3];
     double pmin[3];
+    /* Bins of the grid per reference cell along each axis: cell_size above
+     * is the bin size, the reference's cell (cell.pyx:893-925) is
+     * bins_per_cell bins wide, and the jump check of sph_bin compares
+     * reference cells, floor((lo + bin) / bins_per_cell).  0 or 1: the bins
+     * are the reference's cells.  Searches only read cell_size. */
+    int32_t bins_per_cell;
 } sph_grid;
 
 /* Parameters of the WCSPH operations.
@@ -158,9 +164,10 @@ size_t sph_bin_workspace_bytes(int64_t n, int64_t ncells);
  * layout: perm_dev[p] = the current slot of the particle that goes to slot
  * p (stable: current-slot order inside a cell), cell_dev[p] its linear
  * cell, cell_start_dev[ncells+1].  When prev_cell_dev is not NULL it holds
- * each particle's previous cell in the CURRENT layout; a particle whose cell
- * moved by more than one in any axis is counted into *jump_count_dev (the
- * reference's jump tolerance, cell.pyx:1422-1437).  Reads x,y,z. */
+ * each particle's previous cell in the CURRENT layout; a particle whose
+ * reference cell (see bins_per_cell) moved by more than one in any axis is
+ * counted into *jump_count_dev (the reference's jump tolerance,
+ * cell.pyx:1422-1437).  Reads x,y,z. */
 int32_t sph_bin(void *stream, const sph_view *a, const sph_grid *g,
                 int32_t *perm_dev, int32_t *cell_dev, int32_t *cell_start_dev,
                 const int32_t *prev_cell_dev, int32_t *jump_count_dev,
@@ -294,6 +301,20 @@ int32_t sph_density(void *stream, const sph_grid *g, const sph_params *prm,
                     const sph_view *dst, int32_t nsrc, const sph_view *srcs,
                     double rmax, const float *disp_dev, void *rho_dev,
                     void *p_dev, void *cs_dev, const sph_nlist *nlist_out);
+
+/* sph_density for ndst (1 or 2) destination arrays over the same sources:
+ * the SPHRho (+ TaitEquation) calcs of every array of a step, e.g. the
+ * fluid's and the boundary's (density_funcs.pyx:38-84, eos_funcs.pyx:
+ * 105-116).  When both destinations take the packed FP32-pair scan with
+ * the same epilogue (both or neither with p, cs), one launch serves them:
+ * its blocks cover both destination sets, so the shorter pass fills the GPU
+ * while the longer one drains; otherwise each destination is one
+ * sph_density call.  rho_dev[k], p_dev[k], cs_dev[k] belong to dsts[k]
+ * (p_dev / cs_dev may be NULL); no lists, source caches or exclude_self. */
+int32_t sph_density_multi(void *stream, const sph_grid *g, const sph_params *prm,
+                          int32_t ndst, const sph_view *dsts, int32_t nsrc,
+                          const sph_view *srcs, double rmax, const float *disp_dev,
+                          void *const *rho_dev, void *const *p_dev, void *const *cs_dev);
 
 /* SPHDensityRate over all sources (density_funcs.pyx:89-158, hks off):
  * rate_dev[a] = sum_b m_b (v_a - v_b) . grad_a W(r_a - r_b, (h_a+h_b)/2),

@@ -39,7 +39,7 @@ class SphView(ctypes.Structure):
 class SphGrid(ctypes.Structure):
     _fields_ = [("cell_size", ctypes.c_double), ("lo", ctypes.c_int32 * 3),
                 ("dims", ctypes.c_int32 * 3), ("period", ctypes.c_double * 3),
-                ("pmin", ctypes.c_double * 3)]
+                ("pmin", ctypes.c_double * 3), ("bins_per_cell", ctypes.c_int32)]
 
 
 class SphParams(ctypes.Structure):
@@ -149,6 +149,9 @@ _SIGS = {
     "sph_density": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), _P(SphView),
                                      ctypes.c_int32, _P(SphView), ctypes.c_double, _vp, _vp,
                                      _vp, _vp, _P(SphNlist)]),
+    "sph_density_multi": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphParams), ctypes.c_int32,
+                                           _P(SphView), ctypes.c_int32, _P(SphView),
+                                           ctypes.c_double, _vp, _P(_vp), _P(_vp), _P(_vp)]),
     "sph_neighbor_fill_capped": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
                                                   ctypes.c_double, _vp, _vp, _vp,
                                                   ctypes.c_int64, _vp]),

@@ -111,10 +111,13 @@ class CellManager:
         # relative margin of the cell size over the search radius; a binning
         # stays reusable while particles move less than skin*radius/2
         self.skin = 0.0
-        # True: check at every binning (one read-back); "deferred": count on
-        # the device and check in verify_jumps(); False: no check
+        # True: count on the device at every binning, read back without a
+        # wait and raised at a later binning or in verify_jumps();
+        # "deferred": checked in verify_jumps() only; False: no check
         self.check_jumps = True
         self.cell_size = 0.0
+        # binning grid bins per reference cell (set_cells_per_radius)
+        self.bins_per_cell = 1
         self.min_h = self.max_h = 0.0
         self.is_dirty = True
         self.initialized = False
@@ -125,6 +128,14 @@ class CellManager:
         self.num_binnings = 0
         self._ws = None
         self._jump_acc = None
+        # asynchronous read-backs of the default path (no host waits per
+        # binning): the jump count of check_jumps=True and the bounds that
+        # size the grid, copied to pinned host memory behind an event
+        self.async_bounds = True
+        self._jump_read = None
+        self._jump_seen = 0
+        self._bounds_read = None
+        self._h_versions = None
         if initialize and self.arrays_to_bin:
             self.initialize()
 
@@ -168,15 +179,25 @@ class CellManager:
             self.grid = None
 
     def set_cells_per_radius(self, k):
-        """Cell size = max_radius_scale * max h / k (k = 1 is the reference's
-        rule).  Smaller cells test fewer candidates per neighbour at the
-        price of more stencil rows; neighbour sets do not change."""
-        if self.max_h <= 0:
-            self.compute_cell_size(self.min_cell_size, self.max_cell_size)
-        self.min_cell_size = self.max_radius_scale * self.max_h / float(k)
-        self.initialize()
+        """Bin the particles on a grid k times finer than the reference's
+        cells (k = 1: the bins are the cells).  cell_size, the cell ids it
+        defines and the jump tolerance keep the reference's cell; smaller
+        bins test fewer candidates per neighbour at the price of more
+        stencil rows, and neighbour sets do not change."""
+        k = int(k)
+        if k < 1:
+            raise ValueError("cells per radius must be >= 1, got %d" % k)
+        self.bins_per_cell = k
         self.grid = None
-        return self.cell_size
+        self.is_dirty = True
+        for pa in self.arrays_to_bin:
+            pa.set_dirty(True)
+        return self.cell_size / k if self.cell_size else 0.0
+
+    @property
+    def bin_size(self):
+        """Edge of one bin of the binning grid: cell_size / bins_per_cell."""
+        return (self.cell_size or 1.0) / self.bins_per_cell
 
     def set_fixed_domain(self, lo, hi):
         """Bin on a fixed box (no bounds read-back): needed for graph capture.
@@ -215,8 +236,35 @@ class CellManager:
             hi[bad] = 0.0
         return lo, hi
 
+    def _h_state(self):
+        return tuple(pa.__dict__.get("_h_version", 0) for pa in self.arrays_to_bin)
+
+    def _launch_bounds(self):
+        """Bounds of every array into one device buffer, copied to pinned
+        host memory on the stream behind an event (no wait)."""
+        arrs = [pa for pa in self.arrays_to_bin if pa.get_number_of_particles() > 0]
+        if not arrs:
+            return None
+        dev = arrs[0].device
+        stream = require_gpu(arrs[0].get_carray("x"))
+        buf = torch.empty((len(arrs), _abi.SPH_BOUNDS_WORDS), dtype=torch.float64, device=dev)
+        for k, pa in enumerate(arrs):
+            _abi.call("sph_bounds", stream, pa.view(("x", "y", "z", "h")), buf[k].data_ptr())
+        return _AsyncRead(buf)
+
+    @staticmethod
+    def _reduce_bounds(b):
+        """lo[3], hi[3] and max h over rows of sph_bounds output."""
+        lo = np.min(b[:, :3], axis=0)
+        hi = np.max(b[:, 3:6], axis=0)
+        bad = ~(np.isfinite(lo) & np.isfinite(hi))
+        lo[bad] = 0.0
+        hi[bad] = 0.0
+        return lo, hi, float(np.max(b[:, 6]))
+
     def make_grid(self, lo, hi):
-        g = grid_for_domain(lo, hi, self.cell_size, self.grid_slack)
+        g = grid_for_domain(lo, hi, self.bin_size, self.grid_slack)
+        g.bins_per_cell = self.bins_per_cell
         self._set_periodic(g)
         return g
 
@@ -271,9 +319,22 @@ class CellManager:
         return self._ws
 
     def update(self):
-        """Re-bin dirty arrays (all arrays when the grid changes)."""
+        """Re-bin dirty arrays (all arrays when the grid changes).
+
+        The default path waits on the host for nothing.  The jump check of
+        check_jumps=True counts on the device and its count is read back
+        asynchronously: a violation raises the reference's JumpToleranceError
+        (cell.pyx:1422-1437) at the first binning after the count reached the
+        host, and verify_jumps() (called at the end of Solver.solve) waits
+        for it.  The bounds that size the grid are read asynchronously too:
+        the grid follows the latest bounds that have reached the host, which
+        may be a step old; particles outside the grid are clamped into its
+        border cells, so the neighbour search stays exact.  Bounds are read
+        synchronously at the first binning and whenever an array's h has
+        changed, since h sets the search radius."""
         if not self.initialized:
             self.initialize()
+        self._poll_jumps(wait=False)
         self.update_status()
         if not self.is_dirty and self.grid is not None:
             return 0
@@ -283,9 +344,27 @@ class CellManager:
                     self.wrap_periodic(pa)
         if self.fixed_domain is not None:
             lo, hi = self.fixed_domain
+        elif (self.async_bounds and self.grid is not None and
+              self._h_versions == self._h_state()):
+            lo = hi = None
+            res = self._bounds_read.value() if self._bounds_read is not None else None
+            if res is not None:
+                self._bounds_read = None
+                lo, hi, hmax = self._reduce_bounds(res)
+                if hmax > 0 and hmax != self.max_h:
+                    # h was changed in place on the device: size the grid
+                    # from bounds read now
+                    lo, hi = self._bounds()
+            if self._bounds_read is None:
+                self._bounds_read = self._launch_bounds()
         else:
             lo, hi = self._bounds()
-        g = self.make_grid(lo, hi)
+            self._h_versions = self._h_state()
+            self._bounds_read = None
+        if lo is None:
+            g = self.grid
+        else:
+            g = self.make_grid(lo, hi)
         if self.grid is not None and self.fixed_domain is None:
             # keep the previous grid when it still covers the bounds, so that
             # unchanged arrays keep valid bins and the jump check applies
@@ -334,26 +413,19 @@ class CellManager:
                             pa._cell.numel() == n) else None
         jumps = None
         if prev is not None:
-            if self.check_jumps == "deferred":
-                # accumulate on the device; verify_jumps() reads it later
-                if self._jump_acc is None or self._jump_acc.device != dev:
-                    self._jump_acc = torch.zeros(1, dtype=torch.int32, device=dev)
-                jumps = self._jump_acc
-            else:
-                jumps = torch.zeros(1, dtype=torch.int32, device=dev)
+            # accumulated on the device; "deferred" reads it in
+            # verify_jumps(), True also asynchronously after every binning
+            if self._jump_acc is None or self._jump_acc.device != dev:
+                self._jump_acc = torch.zeros(1, dtype=torch.int32, device=dev)
+            jumps = self._jump_acc
         v = pa.view(("x", "y", "z", "h"))
         if pos is not None:
             v.x, v.y, v.z = (t.data_ptr() for t in pos)
         _abi.call("sph_bin", stream, v, g, perm.data_ptr(), cell.data_ptr(),
                   cell_start.data_ptr(), prev.data_ptr() if prev is not None else None,
                   jumps.data_ptr() if jumps is not None else None, ws.data_ptr(), ws.numel())
-        if jumps is not None and self.check_jumps != "deferred":
-            nj = int(jumps.item())
-            if nj > 0:
-                raise _abi.JumpToleranceError(
-                    "CellManager.update", _abi.SPH_EJUMP,
-                    "%d particle(s) of %r moved by more than one cell width "
-                    "(jump tolerance %d)" % (nj, pa.name, self.jump_tolerance))
+        if jumps is not None and self.check_jumps is True and self._jump_read is None:
+            self._jump_read = _AsyncRead(jumps)
         moved = pa.apply_permutation(perm, extra)
         pa.__dict__["_cell"] = cell
         pa.__dict__["_cell_start"] = cell_start
@@ -363,12 +435,31 @@ class CellManager:
         self.num_binnings += 1
         return moved
 
+    def _poll_jumps(self, wait):
+        """Raise when the device jump count (read back asynchronously, or
+        waited for) has grown since it was last seen."""
+        if self._jump_read is None:
+            return
+        v = self._jump_read.value(wait=wait)
+        if v is None:
+            return
+        self._jump_read = None
+        nj = int(v[0])
+        if nj > self._jump_seen:
+            self._jump_seen = nj
+            raise _abi.JumpToleranceError(
+                "CellManager.update", _abi.SPH_EJUMP,
+                "%d particle move(s) of more than one cell width (jump tolerance %d)"
+                % (nj, self.jump_tolerance))
+
     def verify_jumps(self):
-        """Raise if any deferred jump check counted a particle that moved
-        more than one cell between two binnings."""
-        if self._jump_acc is not None:
+        """Raise if the jump check counted a particle that moved more than
+        one cell between two binnings (waits for the device count)."""
+        self._jump_read = None
+        if self._jump_acc is not None and self.check_jumps:
             nj = int(self._jump_acc.item())
-            if nj > 0:
+            if nj > self._jump_seen:
+                self._jump_seen = nj
                 raise _abi.JumpToleranceError(
                     "CellManager.update", _abi.SPH_EJUMP,
                     "%d particle move(s) of more than one cell width" % nj)
@@ -447,3 +538,22 @@ class CellManager:
                 ids = order[ids]
             out[IntPoint(cx + g.lo[0], cy + g.lo[1], cz + g.lo[2])] = np.sort(ids)
         return out
+
+
+class _AsyncRead:
+    """A small device tensor copied to pinned host memory on the current
+    stream behind an event; value() is the host copy once the copy has
+    completed (None before, unless wait)."""
+
+    def __init__(self, t):
+        self.host = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+        self.host.copy_(t, non_blocking=True)
+        self.event = torch.cuda.Event()
+        self.event.record()
+
+    def value(self, wait=False):
+        if wait:
+            self.event.synchronize()
+        elif not self.event.query():
+            return None
+        return self.host.numpy()

@@ -95,6 +95,7 @@ class ParticleArray:
             info.setdefault("name", nm)
             self.add_property(info)
         self.__dict__["is_dirty"] = True
+        self.__dict__["_h_version"] = self.__dict__.get("_h_version", 0) + 1
         self.align_particles()
 
     def _torch_dtype(self, kind):
@@ -213,6 +214,8 @@ class ParticleArray:
                 self.align_particles()
             if nm in _POS:
                 self.__dict__["is_dirty"] = True
+            if nm == "h":
+                self.__dict__["_h_version"] = self.__dict__.get("_h_version", 0) + 1
 
     # ----------------------------------------------------------- alignment
     def align_particles(self):
@@ -359,6 +362,7 @@ class ParticleArray:
         self._props.clear()
         self._props.update(tensors)
         d = self.__dict__
+        d["_h_version"] = d.get("_h_version", 0) + 1
         d["_n"] = n
         d["_order"] = None
         d["_cell"] = None

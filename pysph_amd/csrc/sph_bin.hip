@@ -190,7 +190,13 @@ struct BinGrid {
   double cs;
   int lo[3];
   int dims[3];
+  int k;  // bins per reference cell along each axis (the jump check's unit)
 };
+
+// floor(a / k) for k >= 1 and any sign of a.
+__device__ __forceinline__ int floor_div(int a, int k) {
+  return a >= 0 ? a / k : -((-a + k - 1) / k);
+}
 
 // floor(x/cs) as the reference computes it (cell.pyx:57), shifted into the
 // grid and clamped.
@@ -248,9 +254,19 @@ __global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
   if (r.is_head) atomicAdd(&counts[c], r.len);
   if (valid && prev_cell != nullptr) {
     const int pc = prev_cell[i];
-    const int px = pc % g.dims[0], t = pc / g.dims[0];
-    const int py = t % g.dims[1], pz = t / g.dims[1];
-    if (abs(px - cx) > 1 || abs(py - cy) > 1 || abs(pz - cz) > 1) atomicAdd(jumps, 1);
+    int px = pc % g.dims[0], t = pc / g.dims[0];
+    int py = t % g.dims[1], pz = t / g.dims[1];
+    int qx = cx, qy = cy, qz = cz;
+    if (g.k > 1) {
+      // compare reference cells, floor(x / cell_size) = floor(bin / k)
+      px = floor_div(px + g.lo[0], g.k);
+      py = floor_div(py + g.lo[1], g.k);
+      pz = floor_div(pz + g.lo[2], g.k);
+      qx = floor_div(qx + g.lo[0], g.k);
+      qy = floor_div(qy + g.lo[1], g.k);
+      qz = floor_div(qz + g.lo[2], g.k);
+    }
+    if (abs(px - qx) > 1 || abs(py - qy) > 1 || abs(pz - qz) > 1) atomicAdd(jumps, 1);
   }
 }
 
@@ -635,6 +651,7 @@ int32_t sph_bin(void* stream, const sph_view* a, const sph_grid* g, int32_t* per
     bg.lo[d] = g->lo[d];
     bg.dims[d] = g->dims[d];
   }
+  bg.k = g->bins_per_cell > 1 ? g->bins_per_cell : 1;
   SPH_TRY(fill_i32(st, counts, ncells, 0));
   const int nb = nblocks_for(n);
   if (n > 0) {

@@ -18,6 +18,30 @@ void density_scan(Grid g, PassArgs<float> a) {
                                           &qbuf[wid][0][lane]);
 }
 
+// The density passes of two destination arrays over the same sources in
+// one launch: blocks [0, nb0) serve a0's destinations, the others a1's.
+// One grid fills the GPU while the longer pass drains, which two launches
+// would need a second stream for.
+template <typename CT, bool TAIT, int DIM>
+__global__ __launch_bounds__(kBlock)
+#if SPH_DENS_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
+#endif
+void density_scan2(Grid g, PassArgs<float> a0, PassArgs<float> a1, int nb0) {
+  __shared__ int qbuf[kBlock / 64][kDensQueue][64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  // a block-uniform branch with one body per argument set: selecting a
+  // reference to either kernel argument at run time would copy them to
+  // scratch memory and read every field from there
+  if (bid < nb0)
+    density_scan_body<CT, TAIT, false, DIM>(g, a0, bid * kBlock + threadIdx.x,
+                                            &qbuf[wid][0][lane]);
+  else
+    density_scan_body<CT, TAIT, false, DIM>(g, a1, (bid - nb0) * kBlock + threadIdx.x,
+                                            &qbuf[wid][0][lane]);
+}
+
 // Density from a list (exact test on every entry); lanes whose list is
 // invalid or overflowed scan the cells.
 template <typename CT, bool TAIT>
@@ -259,6 +283,30 @@ int32_t fast_density(hipStream_t st, const sph_grid& g, double rmax, PassArgs<fl
   else
     density_launch<float>(st, G, a, tait);
   return launch_status("sph_density");
+}
+
+template <int DIM>
+static void density_pair_launch(hipStream_t st, const Grid& G, const PassArgs<float>& a0,
+                                const PassArgs<float>& a1, bool tait) {
+  const int nb0 = nblocks_for(a0.dst.n), nb1 = nblocks_for(a1.dst.n);
+  if (nb0 + nb1 == 0) return;
+  if (tait)
+    hipLaunchKernelGGL((density_scan2<float, true, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0, st,
+                       G, a0, a1, nb0);
+  else
+    hipLaunchKernelGGL((density_scan2<float, false, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0,
+                       st, G, a0, a1, nb0);
+}
+
+int32_t fast_density_pair(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a0,
+                          PassArgs<float> a1, bool tait) {
+  const Grid G = make_grid(g, rmax);
+  switch (scan_dim<float>(a0.dim)) {
+    case 2: density_pair_launch<2>(st, G, a0, a1, tait); break;
+    case 3: density_pair_launch<3>(st, G, a0, a1, tait); break;
+    default: density_pair_launch<0>(st, G, a0, a1, tait); break;
+  }
+  return launch_status("sph_density_multi");
 }
 
 template <typename CT, int MODE>

@@ -184,6 +184,10 @@ static inline bool packed_inputs(const PassArgs<T>& a, bool need_aux) {
 // Fast-path launchers (sph_fast.hip).  rmax is the largest search radius.
 int32_t fast_density(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,
                      bool tait, int pair_prec);
+// The FP32-pair density passes of two destinations over the same sources
+// in one launch (no lists, no source cache).
+int32_t fast_density_pair(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a0,
+                          PassArgs<float> a1, bool tait);
 int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a,
                       int mode, int pair_prec);
 

@@ -587,6 +587,60 @@ int32_t sph_density_cached(void* stream, const sph_grid* g, const sph_params* pr
                        nullptr, cache);
 }
 
+int32_t sph_density_multi(void* stream, const sph_grid* g, const sph_params* prm, int32_t ndst,
+                          const sph_view* dsts, int32_t nsrc, const sph_view* srcs, double rmax,
+                          const float* disp_dev, void* const* rho_dev, void* const* p_dev,
+                          void* const* cs_dev) {
+  if (ndst < 0 || ndst > 2 || (ndst > 0 && (!dsts || !rho_dev))) {
+    set_error("sph_density_multi: ndst must be 0, 1 or 2 with dsts and rho arrays");
+    return SPH_EINVAL;
+  }
+  auto out = [](void* const* a, int k) { return a ? a[k] : nullptr; };
+  for (int k = 0; k < ndst; ++k) {
+    SPH_TRY(check_common(g, prm, &dsts[k], nsrc, srcs, false, "sph_density_multi"));
+    if (dsts[k].n > 0 && rho_dev[k] == nullptr) {
+      set_error("sph_density_multi: NULL rho output of destination %d", k);
+      return SPH_EINVAL;
+    }
+    if ((out(p_dev, k) == nullptr) != (out(cs_dev, k) == nullptr)) {
+      set_error("sph_density_multi: p and cs of destination %d must both be given or NULL", k);
+      return SPH_EINVAL;
+    }
+  }
+  if (prm->exclude_self != 0) {
+    set_error("sph_density_multi: exclude_self is served by sph_density");
+    return SPH_EINVAL;
+  }
+  // one launch for both destinations when both take the FP32-pair packed
+  // scan with the same epilogue; otherwise one sph_density call each
+  if (ndst == 2 && dsts[0].dtype == SPH_F32 && dsts[1].dtype == SPH_F32 &&
+      prm->pair_prec != SPH_PAIR_F64 &&
+      (out(p_dev, 0) != nullptr) == (out(p_dev, 1) != nullptr)) {
+    PassArgs<float> a[2];
+    bool fast = true;
+    for (int k = 0; k < 2; ++k) {
+      a[k] = PassArgs<float>{};
+      SPH_TRY(fill_views<float>(a[k], &dsts[k], nsrc, srcs, "sph_density_multi"));
+      set_params(a[k], prm);
+      a[k].cache_src = -1;
+      a[k].disp = disp_dev;
+      a[k].rho = (float*)rho_dev[k];
+      a[k].p = (float*)out(p_dev, k);
+      a[k].cs = (float*)out(cs_dev, k);
+      fast = fast && packed_inputs(a[k], false) && !a[k].symmetric;
+    }
+    const Grid G = make_grid(*g, rmax);
+    if (fast && !G.periodic) {
+      SPH_TRY(check_period(g, rmax, "sph_density_multi"));
+      return fast_density_pair(as_stream(stream), *g, rmax, a[0], a[1], out(p_dev, 0) != nullptr);
+    }
+  }
+  for (int k = 0; k < ndst; ++k)
+    SPH_TRY(density_entry(stream, g, prm, &dsts[k], nsrc, srcs, rmax, disp_dev, rho_dev[k],
+                          out(p_dev, k), out(cs_dev, k), nullptr, nullptr));
+  return SPH_OK;
+}
+
 int32_t sph_tait(void* stream, const sph_params* prm, const sph_view* a, void* p_dev,
                  void* cs_dev) {
   if (prm == nullptr || a == nullptr || a->n < 0) {

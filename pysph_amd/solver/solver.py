@@ -192,6 +192,8 @@ class Solver:
                 self.dump_output(*self.print_properties)
             if self.execute_commands is not None and self.count % self.command_interval == 0:
                 self.execute_commands(self)
+        # the jump check's device count of the last steps (cell.pyx:1422-1437)
+        self.particles.cell_manager.verify_jumps()
 
     def dump_output(self, *print_properties):
         """One .npz per array: the print properties (or every property when

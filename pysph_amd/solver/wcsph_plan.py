@@ -118,8 +118,22 @@ class WCSPHPlan:
         # waits cost more than the overlap gains
         self.concurrent_density = None
         self._side = None
+        # fused_density: the density passes of two arrays over the same
+        # sources run as one launch whose blocks cover both destination
+        # sets (sph_density_multi); the shorter pass fills the GPU while the
+        # longer one drains, as the second stream did, without cross-stream
+        # waits.  Off, or where it does not apply, the passes launch one by
+        # one (concurrently per concurrent_density)
+        self.fused_density = True
         # the packed fast path serves FP32 storage
         self.use_packed = self.fluid.dtype == torch.float32
+        # the packed passes bin on a grid DEFAULT_CELLS_PER_RADIUS times
+        # finer than the reference's cells (cell_size, cell ids and the jump
+        # tolerance keep the reference's cell): fewer candidates per
+        # neighbour, the same neighbour sets
+        self.cells_per_radius = self.cm.bins_per_cell
+        if self.use_packed:
+            self.set_cells_per_radius(self.DEFAULT_CELLS_PER_RADIUS)
 
     def enable_lists(self, on=True, skin=None, reuse=None):
         """Density passes record neighbour lists that the momentum pass of
@@ -133,8 +147,11 @@ class WCSPHPlan:
             self.list_reuse = bool(reuse)
         return self.use_list
 
+    DEFAULT_CELLS_PER_RADIUS = 2
+
     def set_cells_per_radius(self, k):
-        """Cell size = search radius / k for the shared CellManager."""
+        """Bins of the shared CellManager's grid per reference cell (k = 1:
+        the reference's cells)."""
         self.cm.set_cells_per_radius(k)
         self.cells_per_radius = k
 
@@ -356,8 +373,31 @@ class WCSPHPlan:
         ent["key"] = key
         return st
 
+    def _fuse_density(self):
+        """Whether the density passes of the step's two arrays run as one
+        launch (sph_density_multi): same sources in the same order, packed
+        inputs, no lists, no source cache, FP32 pair arithmetic."""
+        if not self.fused_density or len(self.rho_calcs) != 2 or not self.use_packed:
+            return False
+        if self.use_list or self.cache_static or self.prm.pair_prec == _abi.SPH_PAIR_F64:
+            return False
+        a, b = self.rho_calcs
+        same = [id(x) for x in a.sources] == [id(x) for x in b.sources]
+        tait = {id(a.dest) in self.tait_arrays, id(b.dest) in self.tait_arrays}
+        return same and len(tait) == 1 and len(a.sources) <= _abi.SPH_MAX_SRC
+
     def _density(self, disp=None, override=None, list_mode=0):
         prm = self.prm
+        if self._fuse_density():
+            c0, c1 = self.rho_calcs
+            tait = id(c0.dest) in self.tait_arrays
+            dsts = [c0.dest, c1.dest]
+            _ops.density_multi(self.cm, prm, dsts, c0.sources,
+                               [d.get_carray("rho") for d in dsts],
+                               [d.get_carray("p") if tait else None for d in dsts],
+                               [d.get_carray("cs") if tait else None for d in dsts],
+                               disp=disp, override=override)
+            return
         main = torch.cuda.current_stream()
         side = None
         conc = self.concurrent_density

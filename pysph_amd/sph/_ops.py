@@ -127,6 +127,20 @@ def density(cm, prm, dst, srcs, rho, p=None, cs=None, disp=None, override=None, 
               search_radius(cm, prm.radius_scale), ptr(disp), ptr(rho), ptr(p), ptr(cs), nl)
 
 
+def density_multi(cm, prm, dsts, srcs, rhos, ps, css, disp=None, override=None):
+    """SPHRho (+ Tait where p, cs are given) of up to two destination arrays
+    over the same sources, in one launch when both take the packed FP32
+    scan (sph_density_multi)."""
+    check_same_dtype(list(dsts) + list(srcs))
+    stream = require_gpu(*rhos)
+    vd = views(dsts, override)
+    vs = views(srcs, override)
+    _abi.call("sph_density_multi", stream, cm.grid, prm, len(dsts), vd, len(srcs), vs,
+              search_radius(cm, prm.radius_scale), ptr(disp),
+              _abi.ptr_array([ptr(t) for t in rhos]), _abi.ptr_array([ptr(t) for t in ps]),
+              _abi.ptr_array([ptr(t) for t in css]))
+
+
 def density_rate(cm, prm, dst, srcs, rate, disp=None, override=None):
     """SPHDensityRate over srcs into rate (density_funcs.pyx:89-158)."""
     check_same_dtype([dst] + list(srcs))

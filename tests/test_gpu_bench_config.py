@@ -55,7 +55,9 @@ def case():
     # the configuration bench.py times
     assert plan.use_packed and not plan.use_list and not plan.stage2_rebin
     assert plan.cells_per_radius == 2 and plan.cm.check_jumps == "deferred"
-    assert abs(plan.cm.cell_size - info["h"]) < 1e-6 * info["h"]  # h stored in FP32
+    # bins of h inside the reference's cells of 2h (h stored in FP32)
+    assert abs(plan.cm.bin_size - info["h"]) < 1e-6 * info["h"]
+    assert abs(plan.cm.cell_size - 2 * info["h"]) < 2e-6 * info["h"]
     assert plan.cm.fixed_domain is not None
     assert _ops.DEFAULT_PAIR_PREC == _abi.SPH_PAIR_NATIVE
     assert f.get_number_of_particles() == col[0] * col[1] * col[2]

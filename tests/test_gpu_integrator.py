@@ -188,8 +188,7 @@ def _dam3d_solver(precision="single", packed=True, cells_per_radius=1, seed=4, n
     s.setup_integrator(particles)
     plan = s.integrator.plan
     assert plan is not None
-    if cells_per_radius != 1:
-        plan.set_cells_per_radius(cells_per_radius)
+    plan.set_cells_per_radius(cells_per_radius)
     plan.use_packed = plan.use_packed and packed
     plan.enable_lists(nlist)
     if max_per is not None:

@@ -187,8 +187,73 @@ def test_jump_tolerance_raises():
     if abs(x[0] - pa.get("x")[0]) < 0.2:
         x[0] = pa.get("x")[0] - 0.35
     pa.set(x=x)
+    # the count is made on the device and read back without a wait: the
+    # first binning after it reached the host raises, or verify_jumps()
+    cm.update()
+    with pytest.raises(JumpToleranceError):
+        cm.verify_jumps()
+    cm.verify_jumps()  # reported once
+
+
+def test_jump_tolerance_raises_at_next_binning():
+    import torch
+    from pysph_amd._abi import JumpToleranceError
+    pa = _random_array(500, 4, "double", h=0.05)
+    cm = base.CellManager(arrays_to_bin=[pa])
+    cm.update()
+    x = pa.get("x")
+    x[:10] = np.where(x[:10] > 0.5, x[:10] - 0.35, x[:10] + 0.35)
+    pa.set(x=x)
+    cm.update()
+    torch.cuda.synchronize()
+    pa.set_dirty(True)
     with pytest.raises(JumpToleranceError):
         cm.update()
+
+
+def test_jump_tolerance_counts_reference_cells_on_finer_bins():
+    """With bins half a cell wide the jump check still compares the
+    reference's cells (cell.pyx:1422-1437): a move of two bins into the next
+    cell passes, a move of two cells raises."""
+    from pysph_amd._abi import JumpToleranceError
+    pa = _random_array(500, 4, "double", h=0.04)
+    cm = base.CellManager(arrays_to_bin=[pa], min_cell_size=0.1)
+    cm.set_cells_per_radius(2)
+    assert cm.cell_size == 0.1 and cm.bin_size == 0.05
+    x = pa.get("x")
+    x[0] = 0.3005  # cell 3, bin 6
+    pa.set(x=x)
+    cm.update()
+    cm.verify_jumps()
+    x[0] = 0.4495  # cell 4, bin 8
+    pa.set(x=x)
+    cm.update()
+    cm.verify_jumps()
+    x[0] = 0.6005  # cell 6, bin 12
+    pa.set(x=x)
+    cm.update()
+    with pytest.raises(JumpToleranceError):
+        cm.verify_jumps()
+
+
+def test_stale_grid_keeps_neighbour_sets_exact():
+    """The bounds that size the grid are read back asynchronously, so a
+    binning may use a grid older than the positions: particles beyond it are
+    clamped into its border cells and the neighbour sets stay exact."""
+    pa = _random_array(4000, 9, "double", h=0.05)
+    cm = base.CellManager(arrays_to_bin=[pa])
+    cm.check_jumps = False
+    nm = base.NNPSManager(cm)
+    cm.update()
+    g0 = cm.grid
+    pa.set(x=pa.get("x") + 0.45, y=pa.get("y") * 1.3 - 0.15)
+    cm.update()
+    assert cm.grid is g0  # bounds of the moved particles not read yet
+    off, nb = nm.get_neighbor_particle_locator(pa, pa, 2.0).get_csr()
+    assert cm.grid is g0
+    off_o, nb_o = O.nnps_csr(to_oracle(pa), cell_size=cm.cell_size)
+    _check_rows(off, nb, off_o, nb_o)
+    assert off[-1] > 5 * pa.get_number_of_particles()
 
 
 def test_capped_fill_matches_fill_and_flags_overflow():

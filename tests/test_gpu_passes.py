@@ -409,3 +409,43 @@ def test_eval_accumulates_into_local_and_zeroes_the_rest(precision):
             np.float64)[local] + ref[c][local]
         scale = np.max(np.abs(ref[c][local]))
         assert np.max(np.abs(got[local] - want)) < TOL[precision] * scale * 10, nm
+
+
+def test_density_multi_one_launch_equals_two():
+    """sph_density_multi (the fluid's and the boundary's SPHRho + Tait in one
+    launch) writes the same bits as two sph_density calls, and matches the
+    oracle; the fused WCSPH plan takes it."""
+    f, b, info = gen.dam_break_3d(dx=1.0 / 40, column=(10, 10, 20), tank=(24, 10, 20), seed=7,
+                                  precision="single")
+    particles = base.Particles([f, b])
+    cm = particles.cell_manager
+    cm.set_cells_per_radius(2)
+    cm.update()
+    prm = _ops.make_params(3, co=info["co"], ro=info["rho0"], gamma=7.0)
+    ov = _ops.packed_override([f, b])
+    outs = []
+    for fused in (False, True):
+        res = {pa.name: [torch.full_like(pa.get_carray("rho"), -1.0) for _ in range(3)]
+               for pa in (f, b)}
+        if fused:
+            _ops.density_multi(cm, prm, [f, b], [f, b], [res[pa.name][0] for pa in (f, b)],
+                               [res[pa.name][1] for pa in (f, b)],
+                               [res[pa.name][2] for pa in (f, b)], override=ov)
+        else:
+            for pa in (f, b):
+                _ops.density(cm, prm, pa, [f, b], *res[pa.name], override=ov)
+        torch.cuda.synchronize()
+        outs.append({k: [t.cpu().numpy() for t in v] for k, v in res.items()})
+    for k in outs[0]:
+        for a, c in zip(outs[0][k], outs[1][k]):
+            assert np.array_equal(a, c), k
+    order = f._order_np()
+    rho_f = np.empty(f.get_number_of_particles())
+    rho_f[order] = outs[1]["fluid"][0]
+    rf, rb = to_oracle(f), to_oracle(b)
+    assert max_rel(rho_f, O.sphrho(3, rf, [rf, rb])) < 1e-5
+    s = solver.wcsph_solver(3, co=info["co"], ro=info["rho0"], alpha=0.1, gravity=info["g"],
+                            xsph_eps=0.5)
+    s.setup_integrator(particles)
+    plan = s.integrator.plan
+    assert plan._fuse_density()

@@ -72,7 +72,7 @@ def _stage2_sets(pa, dt):
     plan = s.integrator.plan
     assert plan is not None and plan.use_packed
     plan.set_cells_per_radius(2)
-    assert abs(plan.cm.cell_size - CS) < 1e-6 * CS  # h is stored in FP32
+    assert abs(plan.cm.bin_size - CS) < 1e-6 * CS  # h is stored in FP32
     plan.cm.update()
     b = plan._buffers()
     plan._pack_all()
