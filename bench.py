@@ -37,7 +37,8 @@ def parse():
     ap.add_argument("--scenes", type=int, default=512)
     ap.add_argument("--surrogate-n", type=int, default=60000,
                     help="surrogate2d: fluid particles of the 2D dam break")
-    ap.add_argument("--no-graph", action="store_true", help="batch2d: launch eagerly")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="dam3d, batch2d: launch every step eagerly (no HIP graph replay)")
     ap.add_argument("--no-autotune", action="store_true", help="batch2d: default schedule")
     ap.add_argument("--cube-n", type=int, default=1 << 24)
     ap.add_argument("--steps", type=int, default=1000)
@@ -214,17 +215,24 @@ def time_kernels(s, nf, nsteps):
     import torch
     from pysph_amd import _abi
     timer = LaunchTimer(nf)
-    # a few untimed steps first, without synchronising, so that the launch
-    # queue is ahead of the GPU and no event waits on the host
-    for _ in range(3):
-        s.integrator.integrate(s.dt)
-    _abi.tracer = timer
+    plan = s.integrator.plan
+    # launches are traced one by one: the steps run eagerly here
+    use_graph = plan.use_graph
+    plan.use_graph = False
     try:
-        for _ in range(nsteps):
+        # a few untimed steps first, without synchronising, so that the
+        # launch queue is ahead of the GPU and no event waits on the host
+        for _ in range(3):
             s.integrator.integrate(s.dt)
-        torch.cuda.synchronize()
+        _abi.tracer = timer
+        try:
+            for _ in range(nsteps):
+                s.integrator.integrate(s.dt)
+            torch.cuda.synchronize()
+        finally:
+            _abi.tracer = None
     finally:
-        _abi.tracer = None
+        plan.use_graph = use_graph
     return timer.durations()
 
 
@@ -913,6 +921,7 @@ def main():
                                                      dropin=args.dropin)
     if args.cells_per_radius is None:
         args.cells_per_radius = s.integrator.plan.cells_per_radius
+    s.integrator.plan.use_graph = not args.no_graph
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
@@ -940,6 +949,9 @@ def main():
     gpu_ms = e0.elapsed_time(e1)
     t = wall
     particles.cell_manager.verify_jumps()
+    plan = s.integrator.plan
+    graph_info = {"replays": plan.graph_replays, "captures": plan.graph_captures,
+                  "error": plan.graph_error}
     cons1 = s.conservation()
     p_rho1, p_m1, p_rho_fluid1 = pair_counts(particles, f, b)
     p_rho, p_m = 0.5 * (p_rho0 + p_rho1), 0.5 * (p_m0 + p_m1)
@@ -1015,6 +1027,7 @@ def main():
                    "driver": ("Solver.solve (grid from particle bounds, per-binning jump "
                               "check)" if args.dropin else
                               "integrator steps (fixed domain, deferred jump check)"),
+                   "hip_graph": graph_info,
                    "parallelism": "single GPU"},
         # the kernel with the most GPU time per step, all its launches of a
         # step: algorithmic bytes per launch over the average isolated launch

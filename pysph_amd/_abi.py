@@ -24,6 +24,7 @@ SPH_SRC_XSPH = 2
 SPH_PAIR_NATIVE = 0
 SPH_PAIR_F64 = 1
 SPH_BOUNDS_WORDS = 456
+SPH_SCAN_NOALLOC_MAX_COUNTS = 1 << 27
 
 _vp = ctypes.c_void_p
 

@@ -136,6 +136,10 @@ class CellManager:
         self._jump_seen = 0
         self._bounds_read = None
         self._h_versions = None
+        # set while a step is captured into a HIP graph: binning keeps the
+        # current grid and starts no host read-back (the graph's owner reads
+        # bounds and jump counts outside the graph)
+        self.capturing = False
         if initialize and self.arrays_to_bin:
             self.initialize()
 
@@ -334,7 +338,8 @@ class CellManager:
         changed, since h sets the search radius."""
         if not self.initialized:
             self.initialize()
-        self._poll_jumps(wait=False)
+        if not self.capturing:
+            self._poll_jumps(wait=False)
         self.update_status()
         if not self.is_dirty and self.grid is not None:
             return 0
@@ -344,6 +349,8 @@ class CellManager:
                     self.wrap_periodic(pa)
         if self.fixed_domain is not None:
             lo, hi = self.fixed_domain
+        elif self.capturing and self.grid is not None:
+            lo = hi = None
         elif (self.async_bounds and self.grid is not None and
               self._h_versions == self._h_state()):
             lo = hi = None
@@ -365,16 +372,10 @@ class CellManager:
             g = self.grid
         else:
             g = self.make_grid(lo, hi)
-        if self.grid is not None and self.fixed_domain is None:
+        if self.grid is not None and self.fixed_domain is None and self._covers(self.grid, g):
             # keep the previous grid when it still covers the bounds, so that
             # unchanged arrays keep valid bins and the jump check applies
-            old = self.grid
-            covers = old.cell_size == g.cell_size and all(
-                old.lo[d] <= g.lo[d] + self.grid_slack - 1 and
-                old.lo[d] + old.dims[d] >= g.lo[d] + g.dims[d] - self.grid_slack + 1
-                for d in range(3))
-            if covers:
-                g = old
+            g = self.grid
         if self.periodic_domain is not None:
             self._check_period()
         key = self.grid_key(g)
@@ -385,6 +386,42 @@ class CellManager:
         self.is_dirty = False
         return 0
 
+    def _covers(self, old, g):
+        """Whether grid `old` holds grid `g` (made from newer bounds) with at
+        least one cell of slack left on every side."""
+        return old.cell_size == g.cell_size and all(
+            old.lo[d] <= g.lo[d] + self.grid_slack - 1 and
+            old.lo[d] + old.dims[d] >= g.lo[d] + g.dims[d] - self.grid_slack + 1
+            for d in range(3))
+
+    def grid_holds_bounds(self):
+        """For steps replayed from a captured graph, which keep the grid:
+        False when h changed or the latest bounds that reached the host are
+        not held by the current grid (the next binning outside a graph then
+        makes a new grid from them); otherwise True, and a new asynchronous
+        bounds read is started."""
+        if self.fixed_domain is not None:
+            return self.grid is not None
+        if self.grid is None or self._h_versions != self._h_state():
+            return False
+        if self._bounds_read is not None:
+            res = self._bounds_read.value()
+            if res is None:
+                return True
+            lo, hi, hmax = self._reduce_bounds(res)
+            if (hmax > 0 and hmax != self.max_h) or not self._covers(
+                    self.grid, self.make_grid(lo, hi)):
+                return False
+            self._bounds_read = None
+        self._bounds_read = self._launch_bounds()
+        return True
+
+    def start_jump_read(self):
+        """Start the asynchronous read-back of the device jump count (steps
+        replayed from a graph, whose binnings start none)."""
+        if self.check_jumps is True and self._jump_acc is not None and self._jump_read is None:
+            self._jump_read = _AsyncRead(self._jump_acc)
+
     def bin_array(self, pa, g, pos=None, extra=()):
         """Counting-sort pa by cell on grid g and permute its properties.
         `pos` (three tensors) bins by other positions than x, y, z (a
@@ -394,8 +431,8 @@ class CellManager:
         key = self.grid_key(g)
         ncells = g.dims[0] * g.dims[1] * g.dims[2]
         dev = pa.device
-        cell_start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
         if n == 0:
+            cell_start = torch.empty(ncells + 1, dtype=torch.int32, device=dev)
             stream = require_gpu(cell_start)
             _abi.call("sph_fill_i32", stream, cell_start.data_ptr(), ncells + 1, 0)
             pa.__dict__["_cell"] = torch.empty(0, dtype=torch.int32, device=dev)
@@ -404,8 +441,19 @@ class CellManager:
             pa.set_dirty(False)
             return
         stream = require_gpu(pa.get_carray("x"))
-        perm = torch.empty(n, dtype=torch.int32, device=dev)
-        cell = torch.empty(n, dtype=torch.int32, device=dev)
+        # binning buffers kept per array: the permutation, the cell starts
+        # and two cell arrays (the previous binning's cells are read by the
+        # jump check while the new ones are written)
+        bb = pa.__dict__.get("_bin_bufs")
+        if bb is None or bb["n"] != n or bb["ncells"] != ncells or bb["device"] != dev:
+            bb = {"n": n, "ncells": ncells, "device": dev,
+                  "perm": torch.empty(n, dtype=torch.int32, device=dev),
+                  "cell": [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(2)],
+                  "cell_start": torch.empty(ncells + 1, dtype=torch.int32, device=dev)}
+            pa.__dict__["_bin_bufs"] = bb
+        perm = bb["perm"]
+        cell = bb["cell"][1] if pa._cell is bb["cell"][0] else bb["cell"][0]
+        cell_start = bb["cell_start"]
         ws = self._workspace(n, ncells, dev)
         # no jump check across a periodic wrap (cell.pyx:404-407)
         prev = pa._cell if (self.check_jumps and self.periodic_domain is None and
@@ -424,7 +472,8 @@ class CellManager:
         _abi.call("sph_bin", stream, v, g, perm.data_ptr(), cell.data_ptr(),
                   cell_start.data_ptr(), prev.data_ptr() if prev is not None else None,
                   jumps.data_ptr() if jumps is not None else None, ws.data_ptr(), ws.numel())
-        if jumps is not None and self.check_jumps is True and self._jump_read is None:
+        if (jumps is not None and self.check_jumps is True and self._jump_read is None and
+                not self.capturing):
             self._jump_read = _AsyncRead(jumps)
         moved = pa.apply_permutation(perm, extra)
         pa.__dict__["_cell"] = cell

@@ -534,19 +534,32 @@ class ParticleArray:
         moved = [k for k in self._props if k not in self._unpermuted]
         names = moved + ["__order__"] + ["__extra__"] * len(extra)
         srcs = [self._props[k] for k in moved] + [self._order] + extra
-        dsts = [torch.empty_like(s) for s in srcs]
+        # properties and the order map alternate between two buffers (the
+        # previous layout's buffer receives the next one), so that a
+        # captured step finds its buffers again two binnings later
+        spare = self.__dict__.setdefault("_spare", {})
+
+        def target(name, s):
+            t = spare.get(name)
+            if (name == "__extra__" or t is None or t is s or t.shape != s.shape or
+                    t.dtype != s.dtype or t.device != s.device):
+                t = torch.empty_like(s)
+            return t
+        dsts = [target(k, s) for k, s in zip(names, srcs)]
         esz = (ctypes_int32 * len(srcs))(*[s.element_size() for s in srcs])
         _abi.call("sph_permute", stream, n, perm.data_ptr(), len(srcs),
                   _abi.ptr_array([s.data_ptr() for s in srcs]),
                   _abi.ptr_array([d.data_ptr() for d in dsts]), esz)
         out = []
-        for k, t in zip(names, dsts):
+        for k, s, t in zip(names, srcs, dsts):
             if k == "__order__":
                 self.__dict__["_order"] = t
+                spare[k] = s
             elif k == "__extra__":
                 out.append(t)
             else:
                 self._props[k] = t
+                spare[k] = s
         return out
 
     def view(self, fields=None):

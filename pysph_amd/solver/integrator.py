@@ -216,8 +216,7 @@ class RK2Integrator(Integrator):
 
     def integrate(self, dt):
         if self.plan is not None:
-            self.plan.rk2_step(dt)
-            self.particles.update()
+            self.plan.step(dt)
             return
         self.set_initial_arrays()
         self.do_step(self.calcs, dt)

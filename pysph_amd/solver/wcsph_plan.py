@@ -11,28 +11,33 @@ density):
     PositionStepping on F                      (integrates x, y[, z])
     XSPHCorrection   on F, optional            (integrates x, y[, z])
 
-Per RK2 step the plan issues, on one stream:
+Per RK2 step the plan issues, on one stream (FP32 storage, the defaults):
 
     re-bin F (counting sort + permutation)          sph_bin / sph_permute
-    per array: density + Tait at X                  sph_density
+    pack F's float4 copies                          sph_pack
+    density + Tait of every array at X, one launch  sph_density_multi
     momentum + XSPH + predictor update -> X1, k1    sph_wcsph_stage(1)
-    per array: density + Tait at X1                 sph_density
-With FP32 storage the density pass of each stage also records every
-array's neighbours, and the momentum pass of the stage reads them instead
-of searching the cells again.  Optionally (list_reuse) the stage-1 lists,
-built with a skin, serve stage 2 too while the predictor's displacement
-stays within the skin (checked on the device), and stage 2 keeps the
-binning.
+    pack F's copies from X1                         sph_pack
+    density + Tait of every array at X1             sph_density_multi
     momentum + XSPH + final update -> X (in place)  sph_wcsph_stage(2)
 
-With stage2_rebin, stage 2 re-bins the moving array by the predicted
-positions X1 and carries X, X1 and k1 through the same permutation, as the
-reference rebuilds its neighbour caches after the predictor
-(integrator.py:621-645); by default the packed path keeps the stage-1
-binning and widens the stage-2 stencil by the predictor's displacement
-(same neighbour sets, one binning less per step).  The update
-order inside the stage kernels is the reference's per-calc order
-(mom, gravity, step, xsph), so results match the generic path.
+Stage 2 keeps the stage-1 binning and widens its stencil by the
+predictor's largest displacement, measured on the device by stage 1 (the
+same neighbour sets, one binning less per step); with stage2_rebin, stage
+2 re-bins the moving array by X1 and carries X, X1 and k1 through the same
+permutation, as the reference rebuilds its neighbour caches after the
+predictor (integrator.py:621-645).  The grid's bins are half the
+reference's cells (cells_per_radius = 2).  Optional neighbour lists
+(enable_lists) let the momentum pass read the neighbours its density pass
+recorded.  The update order inside the stage kernels is the reference's
+per-calc order (mom, gravity, step, xsph), so results match the generic
+path.
+
+step() replays the RK2 step and the particle update after it from HIP
+graphs where it can (use_graph): the binning alternates every particle
+property between two buffers, so two graphs, captured once from the two
+buffer parities, alternate and each replay is one step.  Host read-backs
+(bounds, the jump count) stay outside the graphs.
 """
 import ctypes
 
@@ -40,6 +45,7 @@ import torch
 
 from .. import _abi
 from .._device import ptr, require_gpu
+from ..base.step_graph import StepGraphs
 from ..sph import _ops
 from ..sph.funcs import (GravityForce, MomentumEquation, PositionStepping, SPHRho,
                          TaitEquation, XSPHCorrection)
@@ -134,6 +140,11 @@ class WCSPHPlan:
         self.cells_per_radius = self.cm.bins_per_cell
         if self.use_packed:
             self.set_cells_per_radius(self.DEFAULT_CELLS_PER_RADIUS)
+        # use_graph: step() replays captured HIP graphs where the step allows
+        # it (graph_applies()); False launches every step eagerly
+        self.use_graph = True
+        self._sg = StepGraphs(particles.arrays, self.cm, get_extra=self._pk_versions,
+                              set_extra=self._set_pk_versions)
 
     def enable_lists(self, on=True, skin=None, reuse=None):
         """Density passes record neighbour lists that the momentum pass of
@@ -505,6 +516,72 @@ class WCSPHPlan:
             probe("stage2", b, ov)
         self.fluid.set_dirty(True)
         self.steps += 1
+
+    # ------------------------------------------------------------ HIP graphs
+    def graph_applies(self):
+        """Whether step() may replay graphs: the packed scans without lists
+        or source caches, stage 2 on the stage-1 binning, no periodic axes
+        and no functions run by Particles.update()."""
+        pt = self.particles
+        return bool(self.use_graph and self.use_packed and not self.use_list and
+                    not self.stage2_rebin and not self.cache_static and
+                    self.cm.periodic_domain is None and
+                    not getattr(pt, "misc_prop_update_functions", None) and
+                    not getattr(pt, "correction_manager", None))
+
+    @property
+    def graph_replays(self):
+        return self._sg.replays
+
+    @property
+    def graph_captures(self):
+        return self._sg.captures
+
+    @property
+    def graph_error(self):
+        return self._sg.error
+
+    def drop_graphs(self):
+        self._sg.drop()
+
+    def _pk_versions(self):
+        return {k: v["version"] for k, v in self._pk.items()}
+
+    def _set_pk_versions(self, ver):
+        for k, v in ver.items():
+            self._pk[k]["version"] = v
+
+    def _graph_tag(self, dt):
+        """What a captured step bakes in besides buffers and the grid: the
+        time step, the parameters and the plan's switches."""
+        return (float(dt), bytes(self.prm), bytes(self.flags), self.fused_density,
+                self.concurrent_density, self.use_packed, self.stage2_rebin)
+
+    def _eager_step(self, dt):
+        self.rk2_step(dt)
+        self.particles.update()
+
+    def step(self, dt):
+        """One RK2 step and the particle update after it (what
+        RK2Integrator.integrate runs), replayed from a captured HIP graph
+        when graph_applies() and the state is one a graph was captured from,
+        launched eagerly otherwise (the first steps, after a grid change or
+        any change of the arrays' buffers)."""
+        cm = self.cm
+        n0 = self.steps
+        if not self.graph_applies():
+            self._sg.drop()
+            self._eager_step(dt)
+        else:
+            cm._poll_jumps(wait=False)
+            if not cm.grid_holds_bounds():
+                # the next binning makes a new grid: eager, then capture again
+                self._sg.drop()
+                self._eager_step(dt)
+            elif self._sg.step(lambda: self._eager_step(dt), tag=self._graph_tag(dt)):
+                self.particles.update()
+                cm.start_jump_read()
+        self.steps = n0 + 1
 
     def euler_step(self, dt):
         self.cm.update()

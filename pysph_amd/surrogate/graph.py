@@ -71,8 +71,11 @@ class ParticleGraph:
         rmax = 2.0 * cm.max_h
         v = pa.view()
         _abi.call("sph_neighbor_counts", stream, cm.grid, v, v, 2.0, self.counts.data_ptr())
-        _abi.call("sph_scan_counts", stream, self.n, self.counts.data_ptr(),
-                  self.off.data_ptr(), None)
+        # the scan without a device allocation, so that a build can be
+        # captured into a HIP graph
+        scan = ("sph_scan_counts_noalloc" if self.n <= _abi.SPH_SCAN_NOALLOC_MAX_COUNTS
+                else "sph_scan_counts")
+        _abi.call(scan, stream, self.n, self.counts.data_ptr(), self.off.data_ptr(), None)
         _abi.call("sph_neighbor_fill_capped", stream, cm.grid, v, v, 2.0, self.ids.data_ptr(),
                   self.off.data_ptr(), self.nbr.data_ptr(), self.cap,
                   self.overflow.data_ptr())

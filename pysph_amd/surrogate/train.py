@@ -4,12 +4,18 @@
 
 Rollouts: `bases` 2D dam breaks (column width in [0.3, 1.0] m, height in
 [0.5, 2.0] m, 20k-100k fluid particles, drawn from the seed) are advanced by
-the plain fused WCSPH solver; every `spacing` steps one rollout window of
-dt_factor plain steps is recorded: the state at its start and the
-acceleration that carries the fluid velocity across it,
-a = (u(t + k dt) - u(t)) / (k dt), in units of g.  The network is trained on
-those (state, acceleration) pairs with Adam on the GPU (the library's own
-forward and backward kernels).
+the plain fused WCSPH solver.  The surrogate steps positions by leapfrog with
+the step DT = dt_factor x dt: its velocity state is the mean velocity of the
+last step, u_{n-1/2} = (x_n - x_{n-1}) / DT, and one step is
+u_{n+1/2} = u_{n-1/2} + DT a_n, x_{n+1} = x_n + DT u_{n+1/2}.  Every
+`spacing` plain steps one sample is recorded from three plain positions DT
+apart: the state (x_n, u_{n-1/2}) and the acceleration that carries the
+leapfrog step onto the plain solver's next positions,
+a_n = (x_{n+1} - 2 x_n + x_{n-1}) / DT^2, in units of g.  Matching positions
+rather than velocities carries everything the plain step moves particles by
+(the RK2 path, the XSPH correction) into the target.  The network is
+trained with Adam on the GPU (the library's own forward and backward
+kernels).
 """
 import argparse
 import json
@@ -36,11 +42,16 @@ def dam_scene(rng, n_range=(20000, 100000), hdx=1.3):
     return f, b, info
 
 
+def _positions(pa):
+    return get_user(pa, "x").double(), get_user(pa, "y").double()
+
+
 def generate(n_rollouts, bases, seed, dt_factor=4, spacing=100, log=print, n_range=None,
              warmup=1000):
     """Returns a list of bases: dict(graph, info, samples=[(state, target)]).
     Each base first runs `warmup` plain steps (the summation-density free
-    surface starts far from equilibrium)."""
+    surface starts far from equilibrium).  Samples follow the leapfrog
+    convention of the module docstring."""
     rng = np.random.Generator(np.random.PCG64(seed))
     per = max(1, n_rollouts // bases)
     out = []
@@ -48,23 +59,29 @@ def generate(n_rollouts, bases, seed, dt_factor=4, spacing=100, log=print, n_ran
         f, b, info = dam_scene(rng, n_range or (20000, 100000))
         graph = ParticleGraph(combined_array(f, b), info, domain=info["domain"])
         s = plain_solver(f, b, info)
-        nb = b.get_number_of_particles()
-        bstate = {k: get_user(graph.pa, k)[f.get_number_of_particles():] for k in STATE}
+        nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
+        bstate = {k: get_user(graph.pa, k)[nf:] for k in STATE}
+        DT = dt_factor * info["dt"]
         samples = []
         t0 = time.time()
         for _ in range(warmup):
             s.integrator.integrate(info["dt"])
         for r in range(per):
-            for _ in range(spacing - dt_factor):
+            for _ in range(spacing - 2 * dt_factor):
                 s.integrator.integrate(info["dt"])
-            st = {k: torch.cat([get_user(f, k), bstate[k]]) for k in STATE}
-            u0, v0 = get_user(f, "u"), get_user(f, "v")
+            xm, ym = _positions(f)
             for _ in range(dt_factor):
                 s.integrator.integrate(info["dt"])
-            scale = 1.0 / (dt_factor * info["dt"] * 9.81)
-            tgt = torch.zeros((len(st["x"]), 2), dtype=torch.float32, device=st["x"].device)
-            tgt[:len(u0), 0] = (get_user(f, "u") - u0) * scale
-            tgt[:len(u0), 1] = (get_user(f, "v") - v0) * scale
+            x0, y0 = _positions(f)
+            for _ in range(dt_factor):
+                s.integrator.integrate(info["dt"])
+            xp, yp = _positions(f)
+            fl = {"x": x0, "y": y0, "u": (x0 - xm) / DT, "v": (y0 - ym) / DT}
+            st = {k: torch.cat([fl[k].float(), bstate[k]]) for k in STATE}
+            scale = 1.0 / (DT * DT * 9.81)
+            tgt = torch.zeros((nf + nb, 2), dtype=torch.float32, device=x0.device)
+            tgt[:nf, 0] = ((xp - 2.0 * x0 + xm) * scale).float()
+            tgt[:nf, 1] = ((yp - 2.0 * y0 + ym) * scale).float()
             samples.append((st, tgt))
         torch.cuda.synchronize()
         log("base %d: %d fluid + %d boundary particles, W=%.2f H=%.2f, %d rollouts in %.1f s"
@@ -154,7 +171,7 @@ def main(argv=None):
     log("initial val: %s" % json.dumps(evaluate(model, val)))
     hist = train(model, data, a.epochs, lr=a.lr, log=log, val=val)
     model.save(a.out, history=json.dumps(hist), rollouts=a.rollouts, bases=a.bases,
-               seed=a.seed, dt_factor=4)
+               seed=a.seed, dt_factor=4, epochs=a.epochs, lr=a.lr, target="leapfrog")
     log("saved %s (%.1f s total)" % (a.out, time.time() - t0))
     from .evaluate import compare
     baseline = EdgeMLP(seed=a.seed)

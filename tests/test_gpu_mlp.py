@@ -84,7 +84,9 @@ def test_forward_matches_oracle():
     Yb, _ = M.forward(p, X, E, dst, src, g.n, round_bf16=True)
     err, errb = _rel(y, Y), _rel(y, Yb)
     print("forward normwise rel err: %.3e vs FP64, %.3e vs BF16-rounded oracle" % (err, errb))
-    assert errb < 1e-2
+    # measured 1.4e-4 to 3.3e-4 against the BF16-rounded oracle (only the
+    # FP32 summation order differs) and ~4e-3 against plain FP64
+    assert errb < 3e-3
     assert err < 3e-2
     assert np.abs(Y).max() > 1e-3
 
@@ -104,7 +106,9 @@ def test_backward_matches_oracle():
     X, E, dst, src = M.features(pos4, vel4, rho, bt, off, nbr, c)
     p = M.unflatten(theta.astype(np.float64))
     gg = M.unflatten(grads.astype(np.float64))
-    for round_bf16, tol in ((True, 1e-2), (False, 0.2)):
+    # measured <= 3e-4 against the BF16-rounded oracle and 1.6e-2 to 2.7e-2
+    # against plain FP64 (the BF16 rounding of activations and weights)
+    for round_bf16, tol in ((True, 3e-3), (False, 5e-2)):
         Y, cache = M.forward(p, X, E, dst, src, g.n, round_bf16=round_bf16)
         L, dY = M.loss(Y, T, bt)
         gref = M.backward(p, cache, dY)

@@ -3,16 +3,18 @@ from the training and validation seeds 0 and 1000).
 
 - A rollout of 4x plain steps, checked against the plain step every 10
   steps, stays within 1% mean density error of the plain solver at dt, and
-  is far more accurate than the plain solver run at 4x dt (the method
-  without the network).
-- The trained network predicts held-out 4-dt accelerations better than the
-  untrained one (the SPH acceleration alone).
+  is far more accurate than the plain solver run at 4x dt.
+- The trained network's rollout is closer to the plain solver at dt than
+  the same leapfrog stepping with the untrained network (the SPH
+  acceleration alone), in density and in position, and its one-step
+  predictions of held-out accelerations are better.
 - A surrogate whose check fails falls back to the plain step.
 """
 import os
 
 import numpy as np
 import pytest
+import torch
 
 from pysph_amd.surrogate import evaluate as E
 from pysph_amd.surrogate import train as T
@@ -28,6 +30,8 @@ def test_weights_are_committed():
     assert os.path.exists(WEIGHTS)
     m = EdgeMLP.load(WEIGHTS)
     assert int(m.meta["rollouts"]) == 2000
+    assert int(m.meta["bases"]) == 40
+    assert str(m.meta["target"]) == "leapfrog"
 
 
 def test_surrogate_rollout_within_one_percent_and_beats_plain_at_4dt():
@@ -41,6 +45,22 @@ def test_surrogate_rollout_within_one_percent_and_beats_plain_at_4dt():
     # the plain step at 4 dt: diverges, or is at least 10x less accurate
     assert (not r["big_dt_finite"]) or \
         r["big_dt_density_error"] > 10 * r["surrogate_density_error"]
+    # the learned part earns its place: closer to the plain solver at dt
+    # than the untrained network's rollout of the same stepping
+    assert r["surrogate_density_error"] < r["untrained_density_error"]
+    assert r["surrogate_position_error"] < r["untrained_position_error"]
+    assert r["untrained_fallbacks"] == 0
+
+
+@pytest.mark.parametrize("seed", [7002, 7003])
+def test_trained_rollout_beats_untrained_on_held_out_scenes(seed):
+    r = E.compare(EdgeMLP.load(WEIGHTS), seed, nsteps=100, n=40000, warm=1000,
+                  baseline=EdgeMLP())
+    print({k: r[k] for k in ("surrogate_density_error", "untrained_density_error",
+                             "surrogate_position_error", "untrained_position_error")})
+    assert r["fallbacks"] == 0
+    assert r["surrogate_density_error"] < r["untrained_density_error"]
+    assert r["surrogate_position_error"] < r["untrained_position_error"]
 
 
 def test_trained_network_improves_held_out_predictions():
@@ -51,6 +71,28 @@ def test_trained_network_improves_held_out_predictions():
     print("held-out relative acceleration error: trained %.4f, untrained %.4f"
           % (e_tr["rel_err"], e_un["rel_err"]))
     assert e_tr["rel_err"] < e_un["rel_err"]
+
+
+def test_surrogate_graph_replays_give_the_eager_bits():
+    """Full-kick surrogate steps replay HIP graphs (graph build, network,
+    update); over 12 steps with a check at the 10th they give the bits of
+    the eagerly launched steps."""
+    runs = []
+    for use_graph in (False, True):
+        rng = np.random.Generator(np.random.PCG64(7005))
+        f, b, info = T.dam_scene(rng, (20000, 20000))
+        sur = SurrogateWCSPH(f, b, info, EdgeMLP.load(WEIGHTS), check_every=10)
+        sur.use_graph = use_graph
+        runs.append(sur)
+    for k in range(12):
+        for sur in runs:
+            sur.step()
+        a, b = runs[0].state(), runs[1].state()
+        for key in a:
+            assert torch.equal(a[key], b[key]), (k, key)
+    assert runs[0].graphs.replays == 0
+    assert runs[1].graphs.error is None and runs[1].graphs.replays >= 7
+    assert runs[1].stats == runs[0].stats
 
 
 def test_failed_check_falls_back_to_the_plain_step():
