@@ -166,8 +166,9 @@ def kernel_bytes(nf, nb):
         "stage1": 92 * nf + 44 * nb,
         # stage 2: X1 + h,m,rho,p,cs (44) + X0 (24) + k1 (24) read, X (24) written
         "stage2": 116 * nf + 44 * nb,
-        # the fluid's and the boundary's density passes in one launch
-        "density": 52 * nf + 52 * nb,
+        # the fluid's and the boundary's density passes in one launch: the
+        # sources' x,y,z,h,m read once, rho,p,cs of both arrays written
+        "density": 32 * nf + 32 * nb,
     }
 
 
@@ -335,6 +336,35 @@ def measured_traffic(config_key):
     except OSError:
         pass
     return None, "no counter summary"
+
+
+def measured_counters(config_key):
+    """VALU instructions per wave and waves per launch from the rocprofv3
+    SQ counter summary under profiles/, when it was taken on the current
+    sources."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "dam3d_shader_counters.md")
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("counters-json:"):
+                    rec = json.loads(line[len("counters-json:"):])
+                    if rec.get("source_sha256") != source_digest():
+                        return None, "stale (kernel sources changed since the counter runs)"
+                    t = rec.get("configs", {}).get(config_key)
+                    if t is None:
+                        return None, "no counter run for this configuration"
+                    return t, path
+    except OSError:
+        pass
+    return None, "no counter summary"
+
+
+def valu_issue_busy(valu_per_wave, waves, launch_ms):
+    """Fraction of the GPU's VALU issue cycles a launch's instructions fill:
+    a wave64 VALU instruction occupies one of MI355X's 1024 SIMDs (256 CUs x
+    4) for 4 cycles at 2.4 GHz."""
+    return valu_per_wave * waves * 4.0 / (1024 * 2.4e9 * launch_ms * 1e-3)
 
 
 def cpu_baseline(info):
@@ -964,6 +994,7 @@ def main():
     iso = isolated_launch_ms(s, reps=20)
     cfg_key = config_key(args)
     traffic, traffic_src = measured_traffic(cfg_key)
+    counters, counters_src = measured_counters(cfg_key)
     # algorithmic flops: 20 per density pair, 80 per momentum pair
     p_rho_f = 0.5 * (p_rho_fluid0 + p_rho_fluid1)
     p_rho_b = p_rho - p_rho_f
@@ -993,7 +1024,11 @@ def main():
             "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": tr,
             "traffic_over_algorithmic": (tr / (bytes_step / launches)) if tr else None,
             "valu_tflops": flops_step / (ms_step * 1e-3) / 1e12,
-            "valu_frac": flops_step / (ms_step * 1e-3) / 1e12 / FP32_PEAK_TFLOPS})
+            "valu_frac": flops_step / (ms_step * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+            "valu_issue_busy": (valu_issue_busy(counters[passes[0][0]]["valu_per_wave"],
+                                                counters[passes[0][0]]["waves"],
+                                                ms_step / launches)
+                                if counters and passes[0][0] in counters else None)})
     kernels.sort(key=lambda k: -k["ms_per_step"])
     dom = kernels[0]
     out = {
@@ -1039,6 +1074,10 @@ def main():
                      "traffic_source": traffic_src,
                      "bytes_per_launch": dom["bytes_per_launch"],
                      "avg_launch_ms": dom["avg_launch_ms"],
+                     # what bounds these kernels: VALU issue (counters of
+                     # the committed SQ summary x this run's launch time)
+                     "valu_issue_busy": dom["valu_issue_busy"],
+                     "valu_issue_source": counters_src,
                      "launches_per_step": dom["launches_per_step"], "launches_timed": 20},
         "roofline_kernels": kernels,
         "valu": {"kernel": dom["kernel"], "achieved_tflops": dom["valu_tflops"],

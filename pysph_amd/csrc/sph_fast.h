@@ -345,17 +345,20 @@ __device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buff
 // loads U (x, y, z, m) records (pm4, through the bounds-checked buffer view
 // `rs`, one base offset and the slots as immediate offsets), screens them
 // like screen_block (FP32 test, FP64 inside the +-2e-6 band) and adds
-// m W(q) of every accepted one, q = |r_a - r_b| * inv from the screened
+// m 4w(q) of every accepted one, q = |r_a - r_b| * inv from the screened
 // squared distance, in row order (the order a queue would drain them in).
-// Evaluating W for every candidate costs about what queueing the accepted
-// ones costs (about 11 VALU per candidate either way), and it removes the
-// drain loop with its dependent index read, gather and memory wait per
-// pair.  FP32 pair arithmetic only.
+// 4w is four times the cubic spline's shape (kernels.pyx:367-374):
+// (2 - q)^3 for 1 < q < 2, 4 - 6 q^2 + 3 q^3 for q <= 1; the caller scales
+// the row sums by fac / 4 (the normalisation of a source of uniform h is
+// the same for every pair), which saves the per-pair multiplications by the
+// constants (the pass is bound by VALU issue).  Evaluating w for every
+// candidate costs about what queueing the accepted ones costs, and it
+// removes the drain loop with its dependent index read, gather and memory
+// wait per pair.  FP32 pair arithmetic only.
 template <int U>
 __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int j0, int j1,
                                                      const float4& pd, float r2lo, float r2hi,
-                                                     double r2d, float inv, float fac,
-                                                     float& nr) {
+                                                     double r2d, float inv, float& nr) {
   float4 c[U];
   int off = (int)((unsigned)j0 * 16u);
   asm("" : "+v"(off));
@@ -389,13 +392,12 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
 #pragma unroll
   for (int u = 0; u < U; ++u) {
     const float q = __builtin_amdgcn_sqrtf(d2[u]) * inv;
-    // W(q) of kernels.pyx:367-374 without a branch: t = max(2 - q, 0) makes
-    // the outer branch exactly 0 from q = 2 on
+    // t = max(2 - q, 0) makes the outer branch exactly 0 from q = 2 on
     const float t = fmaxf(2.f - q, 0.f);
-    const float outer = 0.25f * t * t * t;
-    const float inner = 1.f - 1.5f * (q * q) * (1.f - 0.5f * q);
-    const float w = q > 1.f ? outer : inner;
-    nr += ok[u] ? c[u].w * (w * fac) : 0.f;
+    const float outer = t * t * t;
+    const float inner = fmaf(q * q, fmaf(3.f, q, -6.f), 4.f);
+    const float w4 = q > 1.f ? outer : inner;
+    if (ok[u]) nr = fmaf(c[u].w, w4, nr);
   }
 }
 
@@ -529,7 +531,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
             const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
                 (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
             do {
-              density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv[s], fac[s],
+              density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv[s],
                                                 nrs[s]);
               j0 += kDensUnroll;
             } while (__any(j0 < j1));
@@ -539,7 +541,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
 #pragma unroll
       for (int s = 0; s < kDirectSrc; ++s) {
         if (s >= a.nsrc) continue;
-        CT nr = (CT)nrs[s];
+        CT nr = (CT)(nrs[s] * (0.25f * fac[s]));
         if (s == a.cache_src && a.cache_mode == 2) {
           if (local) nr = (CT)a.cache_sums[i];
         } else if (s == a.cache_src && a.cache_mode == 1 && valid) {
@@ -587,12 +589,12 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
           int j0 = 0, j1 = 0;
           if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
           while (__any(j0 < j1)) {
-            density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv, fac, nrf);
+            density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv, nrf);
             j0 += kDensUnroll;
           }
         }
       }
-      nr = (CT)nrf;
+      nr = (CT)(nrf * (0.25f * fac));
       if (s == a.cache_src && a.cache_mode == 1 && valid) a.cache_sums[i] = (double)nr;
       rho += nr;
       continue;

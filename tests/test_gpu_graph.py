@@ -95,3 +95,24 @@ def test_default_solver_replays_graphs_and_checks_jumps():
     assert s.count == 10
     assert plan.graph_error is None and plan.graph_replays >= 7
     assert np.all(np.isfinite(f.get("x")))
+
+
+def test_jump_check_raises_under_graph_replay():
+    """The jump check of a replayed step counts on the device inside the
+    graph; the count is read back between replays and raises the reference's
+    JumpToleranceError (cell.pyx:1422-1437) at a following step or at the
+    end of solve()."""
+    from pysph_amd._abi import JumpToleranceError
+    s, f, b, info = _dam(True, True, check_jumps=True)
+    for _ in range(6):
+        s.integrator.integrate(info["dt"])
+    plan = s.integrator.plan
+    assert plan.graph_replays >= 2 and plan.graph_error is None
+    z = f.get("z")
+    z[0] = z[0] + 0.3  # 0.3 m: many cells of 2h = 0.06 m
+    f.set(z=z)
+    with pytest.raises(JumpToleranceError):
+        for _ in range(4):
+            s.integrator.integrate(info["dt"])
+        torch.cuda.synchronize()
+        s.particles.cell_manager.verify_jumps()
