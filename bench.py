@@ -699,8 +699,9 @@ def main_c1(args):
     a dx = 0.025 lattice in the unit circle, h = 1.3 dx, m = dx^2, rho0 = 1,
     u = -100 x, v = 100 y, Tait c0 = 1400, gamma = 7, alpha = 0.01),
     FP64 storage, dt = 1e-5, through the default Particles / Solver objects
-    (the drop-in path: bounds and jump check read back at every binning).
-    steps/s over `steps` steps after `warmup` steps."""
+    (the drop-in path: bounds and the jump count read back without host
+    waits, steps replayed from HIP graphs after the first ones).  steps/s
+    over `steps` steps after `warmup` steps."""
     import torch
     import pysph_amd.base.api as base
     import pysph_amd.solver.api as solver
@@ -735,6 +736,9 @@ def main_c1(args):
     e1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    plan = s.integrator.plan
+    graph_info = {"replays": plan.graph_replays, "captures": plan.graph_captures,
+                  "error": plan.graph_error}
     c1 = s.conservation()
     cm.update()
     _abi.call("sph_neighbor_counts", torch.cuda.current_stream().cuda_stream, cm.grid,
@@ -767,7 +771,8 @@ def main_c1(args):
         "config": {"workload": "2D elliptical drop (C1), FP64, %s, dt = 1e-5, default "
                                "Particles/Solver objects" % args.c1_integrator.upper(),
                    "n": n, "h": 1.3 * 0.025, "c0": 1400.0, "alpha": 0.01,
-                   "pairs_per_step": pairs_per_step, "parallelism": "single GPU"},
+                   "pairs_per_step": pairs_per_step, "hip_graph": graph_info,
+                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": names.get(dom, dom), "pass": dom,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

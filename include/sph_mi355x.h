@@ -173,6 +173,18 @@ int32_t sph_bin(void *stream, const sph_view *a, const sph_grid *g,
                 const int32_t *prev_cell_dev, int32_t *jump_count_dev,
                 void *workspace_dev, size_t workspace_bytes);
 
+/* The jump tolerance check of CellManager.update (cell.pyx:1422-1437) on
+ * the reference's unbounded cells floor(x / cell_size) per axis
+ * (cell.pyx:57, FP64).  refcell_dev[i] holds particle i's cell from the
+ * previous check (INT64_MIN: none) and receives its current one (21 bits per
+ * axis, biased by 2^20, coordinates clamped to +-2^20; INT64_MIN for a
+ * non-finite position); a particle whose cell moved by more than one in any
+ * axis is counted into *jump_count_dev.  The caller keeps refcell_dev in the
+ * particles' slot order (permuting it with them).  Unlike the check of
+ * sph_bin, this one does not depend on the binning grid.  Reads x, y, z. */
+int32_t sph_jump_check(void *stream, const sph_view *a, double cell_size,
+                       int64_t *refcell_dev, int32_t *jump_count_dev);
+
 /* Gather nfields arrays through perm: dst[f][p] = src[f][perm[p]].
  * elem_bytes[f] is 4 or 8.  Host arrays of device pointers; no output may
  * be one of the inputs.  Runs of consecutive sources are copied 16 bytes

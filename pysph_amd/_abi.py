@@ -144,6 +144,7 @@ _SIGS = {
                                          _P(ctypes.c_int64)]),
     "sph_scan_counts_noalloc": (ctypes.c_int32, [_vp, ctypes.c_int64, _vp, _vp,
                                                  _P(ctypes.c_int64)]),
+    "sph_jump_check": (ctypes.c_int32, [_vp, _P(SphView), ctypes.c_double, _vp, _vp]),
     "sph_neighbor_fill": (ctypes.c_int32, [_vp, _P(SphGrid), _P(SphView), _P(SphView),
                                            ctypes.c_double, _vp, _vp, _vp, ctypes.c_int64,
                                            ctypes.c_int64, _P(ctypes.c_int64)]),

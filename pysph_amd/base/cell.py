@@ -26,6 +26,10 @@ def real_to_int(real_val, step):
     return int(math.floor(real_val / step))
 
 
+# "no cell yet" in the jump check's per-particle reference cells
+_REF_NONE = -(1 << 63)
+
+
 def find_cell_id(pnt, cell_size):
     """Cell index of a point (cell.pyx:61-82)."""
     return IntPoint(real_to_int(pnt.x, cell_size), real_to_int(pnt.y, cell_size),
@@ -422,11 +426,11 @@ class CellManager:
         if self.check_jumps is True and self._jump_acc is not None and self._jump_read is None:
             self._jump_read = _AsyncRead(self._jump_acc)
 
-    def bin_array(self, pa, g, pos=None, extra=()):
+    def bin_array(self, pa, g, pos=None, extra=(), extra_out=None):
         """Counting-sort pa by cell on grid g and permute its properties.
         `pos` (three tensors) bins by other positions than x, y, z (a
         predictor stage); `extra` per-particle tensors are permuted along and
-        returned."""
+        returned (into `extra_out` when given)."""
         n = pa.get_number_of_particles()
         key = self.grid_key(g)
         ncells = g.dims[0] * g.dims[1] * g.dims[2]
@@ -442,40 +446,60 @@ class CellManager:
             return
         stream = require_gpu(pa.get_carray("x"))
         # binning buffers kept per array: the permutation, the cell starts
-        # and two cell arrays (the previous binning's cells are read by the
-        # jump check while the new ones are written)
+        # and two cell arrays (each binning writes the one the previous left)
         bb = pa.__dict__.get("_bin_bufs")
         if bb is None or bb["n"] != n or bb["ncells"] != ncells or bb["device"] != dev:
+            rb = bb["ref"] if bb is not None and bb["n"] == n and bb["device"] == dev else [
+                torch.full((n,), _REF_NONE, dtype=torch.int64, device=dev) for _ in range(2)]
             bb = {"n": n, "ncells": ncells, "device": dev,
                   "perm": torch.empty(n, dtype=torch.int32, device=dev),
                   "cell": [torch.empty(n, dtype=torch.int32, device=dev) for _ in range(2)],
-                  "cell_start": torch.empty(ncells + 1, dtype=torch.int32, device=dev)}
+                  "cell_start": torch.empty(ncells + 1, dtype=torch.int32, device=dev),
+                  # the jump check's reference cells (two buffers, as the
+                  # cells); they outlive grid changes
+                  "ref": rb}
             pa.__dict__["_bin_bufs"] = bb
         perm = bb["perm"]
         cell = bb["cell"][1] if pa._cell is bb["cell"][0] else bb["cell"][0]
         cell_start = bb["cell_start"]
         ws = self._workspace(n, ncells, dev)
-        # no jump check across a periodic wrap (cell.pyx:404-407)
-        prev = pa._cell if (self.check_jumps and self.periodic_domain is None and
-                            pa._grid_key == key and pa._cell is not None and
-                            pa._cell.numel() == n) else None
-        jumps = None
-        if prev is not None:
+        v = pa.view(("x", "y", "z", "h"))
+        if pos is not None:
+            v.x, v.y, v.z = (t.data_ptr() for t in pos)
+        extra = list(extra)
+        if extra_out is None:
+            extra_out = [torch.empty_like(t) for t in extra]
+        # the jump check (cell.pyx:1422-1437) on the reference's unbounded
+        # cells, kept per particle and permuted with the particles; none
+        # across a periodic wrap (cell.pyx:404-407)
+        check = bool(self.check_jumps) and self.periodic_domain is None
+        ref = pa.__dict__.get("_refcell")
+        if ref is not bb["ref"][0] and ref is not bb["ref"][1]:
+            ref = bb["ref"][0]
+        ref_next = bb["ref"][1] if ref is bb["ref"][0] else bb["ref"][0]
+        if check:
+            if not bb.get("ref_valid", True):
+                # not kept while the check was off: start again
+                ref.fill_(_REF_NONE)
+            bb["ref_valid"] = True
             # accumulated on the device; "deferred" reads it in
             # verify_jumps(), True also asynchronously after every binning
             if self._jump_acc is None or self._jump_acc.device != dev:
                 self._jump_acc = torch.zeros(1, dtype=torch.int32, device=dev)
-            jumps = self._jump_acc
-        v = pa.view(("x", "y", "z", "h"))
-        if pos is not None:
-            v.x, v.y, v.z = (t.data_ptr() for t in pos)
+            _abi.call("sph_jump_check", stream, v, float(self.cell_size or 1.0), ref.data_ptr(),
+                      self._jump_acc.data_ptr())
+            if self.check_jumps is True and self._jump_read is None and not self.capturing:
+                self._jump_read = _AsyncRead(self._jump_acc)
+        else:
+            bb["ref_valid"] = False
         _abi.call("sph_bin", stream, v, g, perm.data_ptr(), cell.data_ptr(),
-                  cell_start.data_ptr(), prev.data_ptr() if prev is not None else None,
-                  jumps.data_ptr() if jumps is not None else None, ws.data_ptr(), ws.numel())
-        if (jumps is not None and self.check_jumps is True and self._jump_read is None and
-                not self.capturing):
-            self._jump_read = _AsyncRead(jumps)
-        moved = pa.apply_permutation(perm, extra)
+                  cell_start.data_ptr(), None, None, ws.data_ptr(), ws.numel())
+        if check:
+            moved = pa.apply_permutation(perm, [ref] + extra, [ref_next] + list(extra_out))
+            pa.__dict__["_refcell"] = moved[0]
+            moved = moved[1:]
+        else:
+            moved = pa.apply_permutation(perm, extra, extra_out)
         pa.__dict__["_cell"] = cell
         pa.__dict__["_cell_start"] = cell_start
         pa.__dict__["_grid_key"] = key

@@ -71,6 +71,8 @@ class ParticleArray:
         # binning state written by CellManager
         d["_cell"] = None
         d["_cell_start"] = None
+        d["_bin_bufs"] = None
+        d["_refcell"] = None
         d["_grid_key"] = None
         self.initialize(**props)
 
@@ -367,6 +369,8 @@ class ParticleArray:
         d["_order"] = None
         d["_cell"] = None
         d["_cell_start"] = None
+        d["_bin_bufs"] = None
+        d["_refcell"] = None
         d["_grid_key"] = None
         d["is_dirty"] = True
 
@@ -522,10 +526,12 @@ class ParticleArray:
         self._props[nm].copy_(t)
 
     # -------------------------------------------------------- device layout
-    def apply_permutation(self, perm, extra=()):
+    def apply_permutation(self, perm, extra=(), extra_out=None):
         """Gather every property (and the order map) through perm
         (slot p <- slot perm[p]) on the GPU.  `extra` per-particle tensors
-        are gathered along and returned in a new list."""
+        are gathered along and returned in a list: into `extra_out` (tensors
+        of the same shapes, none of them an input) when given, else into
+        new tensors."""
         stream = require_gpu(perm)
         n = self._n
         if self._order is None:
@@ -546,6 +552,14 @@ class ParticleArray:
                 t = torch.empty_like(s)
             return t
         dsts = [target(k, s) for k, s in zip(names, srcs)]
+        if extra_out is not None:
+            extra_out = list(extra_out)
+            if len(extra_out) != len(extra) or any(
+                    o.shape != e.shape or o.dtype != e.dtype or o.device != e.device or
+                    any(o is x for x in extra) for o, e in zip(extra_out, extra)):
+                raise ValueError("extra_out must hold one tensor of the same shape, dtype and "
+                                 "device per extra tensor, none of them an input")
+            dsts[len(dsts) - len(extra):] = extra_out
         esz = (ctypes_int32 * len(srcs))(*[s.element_size() for s in srcs])
         _abi.call("sph_permute", stream, n, perm.data_ptr(), len(srcs),
                   _abi.ptr_array([s.data_ptr() for s in srcs]),

@@ -3,11 +3,14 @@
 A binning permutes every property of an array into the buffer the previous
 binning left (ParticleArray.apply_permutation alternates two buffers per
 property) and writes its cells into the other of two cell arrays, so the
-buffers a step reads return after two steps.  StepGraphs captures two
-consecutive steps, one from each buffer parity, without running them, and
-replays them alternately: each replay is one step.  The Python-side state
-a step leaves (which buffer holds each property, the cell arrays, the grid,
-dirty flags) is recorded at capture and restored after each replay.
+buffers a step reads come back after a few steps (two for an RK2 step;
+six for an Euler step, which also swaps the integrated fields with its
+output buffers, a cycle of three, while the other fields alternate).  StepGraphs captures consecutive steps without running them
+until the buffers are back where they started, one graph per step of that
+cycle, and replays them in turn: each replay is one step.  The
+Python-side state a step leaves (which buffer holds each property, the
+cell arrays, the grid, dirty flags, the owner's own buffers) is recorded
+at capture and restored after each replay.
 
 A replay needs the state it was captured from; any other state (new
 buffers after particles were added, another grid, another dirty pattern)
@@ -17,7 +20,15 @@ manager keeps its grid and starts no host read-back (CellManager.capturing);
 the owner reads bounds and jump counts outside the graphs.  A step that
 cannot be captured (a host synchronisation, an allocation) switches graphs
 off for the owner and records why in `error`.
+
+A replay costs the host one call, so without a limit the host would queue
+a whole run before the GPU finished its first steps, and what the owner
+reads back between steps (the jump count that raises the reference's jump
+tolerance error, the bounds that size the grid) would arrive only at the
+end.  The host therefore waits until the GPU has finished the replay
+`max_lead` steps back before it queues the next one.
 """
+import collections
 import ctypes
 
 import torch
@@ -27,12 +38,20 @@ from .._device import stream_handle
 
 
 class StepGraphs:
-    def __init__(self, arrays, cell_manager, get_extra=None, set_extra=None, warmup=2):
+    def __init__(self, arrays, cell_manager, get_extra=None, set_extra=None, key_extra=None,
+                 warmup=2, max_period=6, max_lead=8):
         self.arrays = list(arrays)
         self.cm = cell_manager
+        # the owner's state: get_extra() is restored by set_extra(); its
+        # key_extra() (hashable: the identities of buffers the step reads)
+        # is part of the state's key
         self.get_extra = get_extra
         self.set_extra = set_extra
+        self.key_extra = key_extra
         self.warmup = int(warmup)
+        self.max_period = int(max_period)
+        self.max_lead = int(max_lead)
+        self._events = collections.deque()
         self.enabled = True
         self.replays = 0
         self.captures = 0
@@ -48,21 +67,23 @@ class StepGraphs:
             d = pa.__dict__
             arrays.append((pa, dict(pa._props), dict(d.get("_spare", {})), d.get("_order"),
                            d.get("_cell"), d.get("_cell_start"), d.get("_grid_key"),
-                           d.get("_bin_version", 0), d.get("is_dirty")))
+                           d.get("_bin_version", 0), d.get("is_dirty"), d.get("_refcell")))
         extra = self.get_extra() if self.get_extra else None
+        xkey = self.key_extra() if self.key_extra else None
         return {"grid": self.cm.grid, "cm_dirty": self.cm.is_dirty, "arrays": arrays,
-                "extra": extra}
+                "extra": extra, "xkey": xkey}
 
     def set_state(self, st):
         cm = self.cm
         cm.grid, cm.is_dirty = st["grid"], st["cm_dirty"]
-        for pa, props, spare, order, cell, cstart, key, ver, dirty in st["arrays"]:
+        for pa, props, spare, order, cell, cstart, key, ver, dirty, ref in st["arrays"]:
             pa._props.clear()
             pa._props.update(props)
             d = pa.__dict__
             d["_spare"] = dict(spare)
             d["_order"], d["_cell"], d["_cell_start"] = order, cell, cstart
             d["_grid_key"], d["_bin_version"], d["is_dirty"] = key, ver, dirty
+            d["_refcell"] = ref
         if self.set_extra:
             self.set_extra(st["extra"])
 
@@ -72,11 +93,12 @@ class StepGraphs:
         step reads (versions and counters excluded)."""
         g = st["grid"]
         key = [None if g is None else (g.cell_size, tuple(g.lo), tuple(g.dims))]
-        for pa, props, spare, order, cell, cstart, gkey, _, dirty in st["arrays"]:
+        for pa, props, spare, order, cell, cstart, gkey, _, dirty, ref in st["arrays"]:
             key.append((id(pa), pa.get_number_of_particles(),
                         tuple(sorted((k, id(t)) for k, t in props.items())),
                         tuple(sorted((k, id(t)) for k, t in spare.items())),
-                        id(order), id(cell), id(cstart), gkey, bool(dirty)))
+                        id(order), id(cell), id(cstart), gkey, bool(dirty), id(ref)))
+        key.append(st["xkey"])
         return tuple(key)
 
     # ---------------------------------------------------------------- graphs
@@ -106,10 +128,11 @@ class StepGraphs:
         lib = _abi.load()
         graphs, states, err = [], [st0], None
         self.cm.capturing = True
+        key0 = self.key(st0)
         try:
             with torch.cuda.stream(gs):
                 hs = gs.cuda_stream
-                for _ in range(2):
+                while len(graphs) < self.max_period:
                     _abi.call("sph_graph_begin", hs)
                     try:
                         fn()
@@ -123,13 +146,16 @@ class StepGraphs:
                     _abi.call("sph_graph_end", hs, ctypes.byref(g))
                     graphs.append(g)
                     states.append(self.state())
+                    if self.key(states[-1]) == key0:
+                        break
         except Exception as e:  # not capturable: run eagerly from now on
             err = e
         finally:
             self.cm.capturing = False
             self.set_state(st0)
-        if err is None and self.key(states[2]) != self.key(st0):
-            err = RuntimeError("the step's buffers do not return after two binnings")
+        if err is None and self.key(states[-1]) != key0:
+            err = RuntimeError("the step's buffers do not come back within %d steps"
+                               % self.max_period)
         if err is not None:
             for g in graphs:
                 lib.sph_graph_destroy(g)
@@ -137,7 +163,7 @@ class StepGraphs:
             self.enabled = False
             return False
         self._g = {"tag": tag, "exec": graphs, "states": states,
-                   "keys": [self.key(states[0]), self.key(states[1])]}
+                   "keys": [self.key(s) for s in states[:-1]]}
         self.captures += 1
         return True
 
@@ -170,4 +196,9 @@ class StepGraphs:
         _abi.call("sph_graph_launch", self._g["exec"][i], stream_handle())
         self.set_state(self._g["states"][i + 1])
         self.replays += 1
+        ev = torch.cuda.Event()
+        ev.record()
+        self._events.append(ev)
+        while len(self._events) > self.max_lead:
+            self._events.popleft().synchronize()
         return True

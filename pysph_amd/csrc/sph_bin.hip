@@ -270,6 +270,53 @@ __global__ __launch_bounds__(kBlock) void bin_count(View<T> a, BinGrid g,
   }
 }
 
+// The jump check of CellManager.update (cell.pyx:1422-1437) on the
+// reference's unbounded cells: the cell of a particle is floor(x/cs) per
+// axis (cell.pyx:57, FP64), kept per particle between checks as one int64
+// (21 bits per axis, biased by 2^20, coordinates clamped to +-2^20; a
+// non-finite position has no cell).  Counting on these rather than on the
+// grid's bins keeps the check independent of the binning grid: of its bin
+// size, of grid changes, and of the clamping of particles outside it into
+// its border bins.
+constexpr int64_t kRefBias = (int64_t)1 << 20;
+constexpr int64_t kRefNone = INT64_MIN;
+
+__device__ __forceinline__ int64_t ref_coord(double x, double cs) {
+  const double f = floor(x / cs);
+  const double c = fmin(fmax(f, (double)-kRefBias), (double)(kRefBias - 1));
+  return (int64_t)c + kRefBias;
+}
+
+__device__ __forceinline__ bool ref_moved(int64_t a, int64_t b) {
+  const int64_t m = (int64_t)(1 << 21) - 1;
+  bool jump = false;
+#pragma unroll
+  for (int s = 0; s < 3; ++s) {
+    const int64_t d = ((a >> (21 * s)) & m) - ((b >> (21 * s)) & m);
+    jump = jump || d > 1 || d < -1;
+  }
+  return jump;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void jump_check_kernel(View<T> a, double cs,
+                                                            int64_t* __restrict__ ref,
+                                                            int* __restrict__ jumps) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  bool jump = false;
+  if (i < a.n) {
+    const double x = (double)a.x[i], y = (double)a.y[i], z = (double)a.z[i];
+    int64_t code = kRefNone;
+    if (isfinite(x) && isfinite(y) && isfinite(z))
+      code = (ref_coord(x, cs) << 42) | (ref_coord(y, cs) << 21) | ref_coord(z, cs);
+    const int64_t prev = ref[i];
+    jump = prev != kRefNone && code != kRefNone && ref_moved(prev, code);
+    ref[i] = code;
+  }
+  const unsigned long long b = __ballot(jump);
+  if ((threadIdx.x & 63) == 0 && b != 0ull) atomicAdd(jumps, (int)__popcll(b));
+}
+
 // Multi-scene binning: particle i belongs to scene scene[i], whose grid
 // occupies cells [cell_base, cell_base + nx*ny*nz) of one global cell space,
 // so one counting sort bins every scene and keeps scenes contiguous.
@@ -672,6 +719,26 @@ int32_t sph_bin(void* stream, const sph_view* a, const sph_grid* g, int32_t* per
   hipLaunchKernelGGL(bin_rank, dim3(nb), dim3(kBlock), 0, st, (int)n, cell_tmp,
                      cell_start_dev, bucket, perm_dev, cell_dev);
   return launch_status("sph_bin");
+}
+
+int32_t sph_jump_check(void* stream, const sph_view* a, double cell_size, int64_t* refcell_dev,
+                       int32_t* jump_count_dev) {
+  SPH_TRY(check_view(a, "sph_jump_check", false, false, false));
+  const int64_t n = a->n;
+  if (!(cell_size > 0.0) || (n > 0 && (!refcell_dev || !jump_count_dev))) {
+    set_error("sph_jump_check: bad arguments (cell size %g)", cell_size);
+    return SPH_EINVAL;
+  }
+  if (n == 0) return SPH_OK;
+  hipStream_t st = as_stream(stream);
+  const int nb = nblocks_for(n);
+  if (a->dtype == SPH_F32)
+    hipLaunchKernelGGL(jump_check_kernel<float>, dim3(nb), dim3(kBlock), 0, st,
+                       to_view<float>(*a), cell_size, refcell_dev, jump_count_dev);
+  else
+    hipLaunchKernelGGL(jump_check_kernel<double>, dim3(nb), dim3(kBlock), 0, st,
+                       to_view<double>(*a), cell_size, refcell_dev, jump_count_dev);
+  return launch_status("sph_jump_check");
 }
 
 int32_t sph_permute(void* stream, int64_t n, const int32_t* perm_dev, int32_t nfields,

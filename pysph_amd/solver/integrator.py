@@ -184,8 +184,7 @@ class EulerIntegrator(Integrator):
 
     def integrate(self, dt):
         if self.plan is not None:
-            self.plan.euler_step(dt)
-            self.particles.update()
+            self.plan.step(dt, euler=True)
             return
         self.set_initial_arrays()
         self.eval(self.calcs)

@@ -143,8 +143,8 @@ class WCSPHPlan:
         # use_graph: step() replays captured HIP graphs where the step allows
         # it (graph_applies()); False launches every step eagerly
         self.use_graph = True
-        self._sg = StepGraphs(particles.arrays, self.cm, get_extra=self._pk_versions,
-                              set_extra=self._set_pk_versions)
+        self._sg = StepGraphs(particles.arrays, self.cm, get_extra=self._graph_extra,
+                              set_extra=self._set_graph_extra, key_extra=self._graph_xkey)
 
     def enable_lists(self, on=True, skin=None, reuse=None):
         """Density passes record neighbour lists that the momentum pass of
@@ -261,6 +261,8 @@ class WCSPHPlan:
                 "n": n, "dtype": pa.dtype,
                 "s1": [mk() for _ in range(6)],
                 "k1": [mk() for _ in range(6)],
+                # the second set X1 and k1 alternate with at a re-binning
+                "alt": [mk() for _ in range(12)],
                 "disp": torch.zeros(1, dtype=torch.float32, device=pa.device),
             }
         return self._bufs
@@ -414,7 +416,9 @@ class WCSPHPlan:
         conc = self.concurrent_density
         if conc is None:
             conc = self.fluid.get_number_of_particles() >= self.CONCURRENT_MIN
-        if conc and len(self.rho_calcs) > 1:
+        # a captured step stays on one stream (its launches are replayed in
+        # order on the launch stream)
+        if conc and len(self.rho_calcs) > 1 and not self.cm.capturing:
             if self._side is None:
                 self._side = torch.cuda.Stream(device=self.fluid.device)
             side = self._side
@@ -455,14 +459,18 @@ class WCSPHPlan:
 
     # ----------------------------------------------------------------- steps
     def rebin_stage(self, b):
-        """Re-bin the moving array by X1; X, X1 and k1 follow the new order."""
+        """Re-bin the moving array by X1; X, X1 and k1 follow the new order
+        (X1 and k1 permuted into the step buffers' second set, which the
+        next re-binning takes back)."""
         s0 = self._state0()
         # components >= dim are not integrated: X1 keeps those of X
         pos = [b["s1"][c] if c < self.dim else s0[c] for c in range(3)]
-        moved = self.cm.bin_array(self.fluid, self.cm.grid, pos=pos,
-                                  extra=b["s1"] + b["k1"])
+        old = b["s1"] + b["k1"]
+        moved = self.cm.bin_array(self.fluid, self.cm.grid, pos=pos, extra=old,
+                                  extra_out=b["alt"])
         b["s1"] = moved[:6]
         b["k1"] = moved[6:]
+        b["alt"] = old
 
     def rk2_step(self, dt, probe=None):
         """One RK2 step.  `probe` (optional) is called as probe(phase, bufs,
@@ -519,12 +527,11 @@ class WCSPHPlan:
 
     # ------------------------------------------------------------ HIP graphs
     def graph_applies(self):
-        """Whether step() may replay graphs: the packed scans without lists
-        or source caches, stage 2 on the stage-1 binning, no periodic axes
-        and no functions run by Particles.update()."""
+        """Whether step() may replay graphs: no neighbour lists or source
+        caches (their validity is decided on the host), no periodic axes and
+        no functions run by Particles.update()."""
         pt = self.particles
-        return bool(self.use_graph and self.use_packed and not self.use_list and
-                    not self.stage2_rebin and not self.cache_static and
+        return bool(self.use_graph and not self.use_list and not self.cache_static and
                     self.cm.periodic_domain is None and
                     not getattr(pt, "misc_prop_update_functions", None) and
                     not getattr(pt, "correction_manager", None))
@@ -544,12 +551,23 @@ class WCSPHPlan:
     def drop_graphs(self):
         self._sg.drop()
 
-    def _pk_versions(self):
-        return {k: v["version"] for k, v in self._pk.items()}
+    def _graph_extra(self):
+        b = self._bufs
+        bufs = None if b is None else {k: list(b[k]) for k in ("s1", "k1", "alt")}
+        return {"pk": {k: v["version"] for k, v in self._pk.items()}, "bufs": bufs}
 
-    def _set_pk_versions(self, ver):
-        for k, v in ver.items():
+    def _set_graph_extra(self, ex):
+        for k, v in ex["pk"].items():
             self._pk[k]["version"] = v
+        if ex["bufs"] is not None and self._bufs is not None:
+            for k, v in ex["bufs"].items():
+                self._bufs[k] = list(v)
+
+    def _graph_xkey(self):
+        b = self._bufs
+        if b is None:
+            return None
+        return tuple(id(t) for k in ("s1", "k1", "alt") for t in b[k])
 
     def _graph_tag(self, dt):
         """What a captured step bakes in besides buffers and the grid: the
@@ -557,28 +575,32 @@ class WCSPHPlan:
         return (float(dt), bytes(self.prm), bytes(self.flags), self.fused_density,
                 self.concurrent_density, self.use_packed, self.stage2_rebin)
 
-    def _eager_step(self, dt):
-        self.rk2_step(dt)
+    def _eager_step(self, dt, euler=False):
+        if euler:
+            self.euler_step(dt)
+        else:
+            self.rk2_step(dt)
         self.particles.update()
 
-    def step(self, dt):
-        """One RK2 step and the particle update after it (what
-        RK2Integrator.integrate runs), replayed from a captured HIP graph
-        when graph_applies() and the state is one a graph was captured from,
-        launched eagerly otherwise (the first steps, after a grid change or
-        any change of the arrays' buffers)."""
+    def step(self, dt, euler=False):
+        """One RK2 (or Euler) step and the particle update after it (what
+        RK2Integrator.integrate / EulerIntegrator.integrate run), replayed
+        from a captured HIP graph when graph_applies() and the state is one
+        a graph was captured from, launched eagerly otherwise (the first
+        steps, after a grid change or any change of the arrays' buffers)."""
         cm = self.cm
         n0 = self.steps
         if not self.graph_applies():
             self._sg.drop()
-            self._eager_step(dt)
+            self._eager_step(dt, euler)
         else:
             cm._poll_jumps(wait=False)
             if not cm.grid_holds_bounds():
                 # the next binning makes a new grid: eager, then capture again
                 self._sg.drop()
-                self._eager_step(dt)
-            elif self._sg.step(lambda: self._eager_step(dt), tag=self._graph_tag(dt)):
+                self._eager_step(dt, euler)
+            elif self._sg.step(lambda: self._eager_step(dt, euler),
+                               tag=self._graph_tag(dt) + (euler,)):
                 self.particles.update()
                 cm.start_jump_read()
         self.steps = n0 + 1

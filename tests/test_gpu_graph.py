@@ -116,3 +116,35 @@ def test_jump_check_raises_under_graph_replay():
             s.integrator.integrate(info["dt"])
         torch.cuda.synchronize()
         s.particles.cell_manager.verify_jumps()
+
+
+@pytest.mark.parametrize("integrator", ["rk2", "euler"])
+def test_graph_replays_of_the_fp64_soa_step(integrator):
+    """FP64 storage (the reference's default precision) runs the SoA
+    kernels, re-binning by the predicted positions in RK2 stage 2 (X1 and
+    k1 alternate between two buffer sets) and swapping the state with the
+    stage output in Euler: the buffers come back after two (RK2) or six
+    (Euler: the swapped fields cycle through three buffers, the others
+    through two) steps, and every replayed step gives the eager bits."""
+    runs = []
+    for use_graph in (False, True):
+        pa = gen.elliptical_drop(dx=0.05, precision="double")
+        particles = base.Particles([pa])
+        s = solver.wcsph_solver(2, integrator=integrator, co=1400.0, ro=1.0, gamma=7.0,
+                                alpha=0.01, beta=0.0, eta=0.1, with_solid=False)
+        s.setup_integrator(particles)
+        plan = s.integrator.plan
+        assert plan is not None and not plan.use_packed
+        plan.use_graph = use_graph
+        runs.append((s, pa, plan))
+    for k in range(12):
+        for s, pa, plan in runs:
+            s.integrator.integrate(1e-5)
+        torch.cuda.synchronize()
+        a, b = _state(runs[0][1]), _state(runs[1][1])
+        for key in a:
+            assert np.array_equal(a[key], b[key]), (k, key)
+    plan = runs[1][2]
+    assert plan.graph_error is None, plan.graph_error
+    assert plan.graph_captures == 1 and plan.graph_replays >= 8
+    assert len(plan._sg._g["exec"]) == (2 if integrator == "rk2" else 6)
