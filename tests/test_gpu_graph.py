@@ -148,3 +148,28 @@ def test_graph_replays_of_the_fp64_soa_step(integrator):
     assert plan.graph_error is None, plan.graph_error
     assert plan.graph_captures == 1 and plan.graph_replays >= 8
     assert len(plan._sg._g["exec"]) == (2 if integrator == "rk2" else 6)
+
+
+def test_graph_replays_after_particles_are_added():
+    """Particles added between steps give the arrays new buffers: the plan
+    steps eagerly, captures again, and every step keeps the eager bits."""
+    runs = [_dam(False, True), _dam(True, True)]
+    for k in range(16):
+        if k == 7:
+            for s, f, b, info in runs:
+                x = f.get("x")
+                sel = slice(0, 40)
+                # a layer dropped above the column (and above the fixed
+                # domain: clamped into its border bins)
+                f.add_particles(x=x[sel], y=f.get("y")[sel], z=f.get("z")[sel] + 0.6,
+                                h=f.get("h")[sel], m=f.get("m")[sel], rho=f.get("rho")[sel])
+        for s, f, b, info in runs:
+            s.integrator.integrate(info["dt"])
+        torch.cuda.synchronize()
+        a, b2 = _state(runs[0][1], runs[0][2]), _state(runs[1][1], runs[1][2])
+        for key in a:
+            assert np.array_equal(a[key], b2[key]), (k, key)
+    plan = runs[1][0].integrator.plan
+    assert plan.graph_error is None, plan.graph_error
+    assert plan.graph_captures == 2
+    assert runs[1][1].get_number_of_particles() == runs[0][1].get_number_of_particles()
