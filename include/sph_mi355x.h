@@ -134,8 +134,10 @@ typedef struct sph_params {
      * inputs and neighbour lists are ignored); the batched engine rejects it. */
     int32_t symmetric;
     /* Bit s set: the summation density drops the self pair of source s, which
-     * must be the destination array (SPHFunctionParticle exclude_self,
-     * sph_func.pyx:386-389, where the self particle is the last neighbour).
+     * must be the destination array: the intended semantics of
+     * SPHFunctionParticle exclude_self (sph_func.pyx:381-394), whose own code
+     * reads an attribute its class does not declare and so never takes
+     * effect; no reference output pins this flag.
      * Read by sph_density / sph_density_cached only: in the other passes
      * the self pair contributes 0.  The density then runs its SoA kernels;
      * the batched engine rejects it. */

@@ -33,7 +33,8 @@ class SPHRho(CSPHFunctionParticle):
     @classmethod
     def calc_eval(cls, calc, outputs):
         cm = grid_of(calc)
-        # sources whose self pair is dropped (sph_func.pyx:386-389)
+        # sources whose self pair is dropped: the intended semantics of
+        # exclude_self (sph_func.pyx:381-394; see SPHFunctionParticle)
         excl = sum(1 << s for s, (f, src) in enumerate(zip(calc.funcs, calc.sources))
                    if getattr(f, "exclude_self", False) and src is calc.dest)
         prm = _ops.calc_params(calc, exclude_self=excl)
