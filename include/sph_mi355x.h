@@ -83,6 +83,11 @@ typedef struct sph_view {
      * pair (same arithmetic, same bits). */
     const void *pm4;
     const int32_t *hrange;
+    /* Optional mass-uniformity pair from sph_pack (like hrange, for m):
+     * when a density source of uniform h also carries pos3 and its mass is
+     * uniform, the density pass reads the 12-byte (x,y,z) records and takes
+     * m = mrange[0] (same products, same bits). */
+    const int32_t *mrange;
 } sph_view;
 
 /* The cell grid shared by all arrays of one Particles set.  Cell (i,j,k) of
@@ -209,11 +214,12 @@ int32_t sph_periodic_wrap(void *stream, const sph_grid *g, int32_t dtype, int64_
 
 /* Write the packed copies pos4[i] = (x,y,z,h), vel4[i] = (u,v,w,m),
  * pm4[i] = (x,y,z,m) (float4 each) and pos3[i] = (x,y,z) (3 floats) of an
- * FP32 array, and its h-uniformity pair hrange[0..1] (the bit pattern of
- * h[0] twice when every h equals it; otherwise hrange[1] holds another h);
- * any output may be NULL. */
+ * FP32 array, its h-uniformity pair hrange[0..1] (the bit pattern of h[0]
+ * twice when every h equals it; otherwise hrange[1] holds another h) and
+ * its mass-uniformity pair mrange[0..1] (the same for m); any output may be
+ * NULL. */
 int32_t sph_pack(void *stream, const sph_view *a, void *pos4_dev, void *vel4_dev,
-                 void *pos3_dev, void *pm4_dev, int32_t *hrange_dev);
+                 void *pos3_dev, void *pm4_dev, int32_t *hrange_dev, int32_t *mrange_dev);
 
 /* ---- FixedDestNbrParticleLocator.get_nearest_particles (nnps.pyx:458-620)
  *      as a CSR export: the exact FP64 test

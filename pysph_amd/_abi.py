@@ -34,7 +34,8 @@ class SphView(ctypes.Structure):
                 ("reserved", ctypes.c_int32)] + [
         (f, _vp) for f in ("x", "y", "z", "u", "v", "w", "h", "m", "rho", "p", "cs")] + [
         ("tag", _vp), ("cell", _vp), ("cell_start", _vp), ("pos4", _vp), ("vel4", _vp),
-        ("aux4", _vp), ("pos3", _vp), ("pm4", _vp), ("hrange", _vp)]
+        ("aux4", _vp), ("pos3", _vp), ("pm4", _vp), ("hrange", _vp),
+        ("mrange", _vp)]
 
 
 class SphGrid(ctypes.Structure):
@@ -186,7 +187,7 @@ _SIGS = {
     "sph_conservation_f64": (ctypes.c_int32, [_vp, _P(SphParams), _P(SphView), _vp]),
     "sph_move_circle": (ctypes.c_int32, [_vp, _P(SphView), ctypes.c_int32, _vp, _vp]),
     "sph_fill_i32": (ctypes.c_int32, [_vp, _vp, ctypes.c_int64, ctypes.c_int32]),
-    "sph_pack": (ctypes.c_int32, [_vp, _P(SphView), _vp, _vp, _vp, _vp, _vp]),
+    "sph_pack": (ctypes.c_int32, [_vp, _P(SphView), _vp, _vp, _vp, _vp, _vp, _vp]),
     "sph_batch_ncells": (ctypes.c_int64, [ctypes.c_int32, _P(SphBatchScene)]),
     "sph_batch_create": (ctypes.c_int32, [ctypes.c_int32, _P(SphBatchScene),
                                           _P(SphBatchBuffers), _P(_vp)]),

@@ -211,6 +211,7 @@ struct View {
   const float* pos3;   // packed (x,y,z), 3 floats per particle
   const float4* pm4;   // packed (x,y,z,m)
   const int* hrange;   // h-uniformity pair (FP32 bits; equal: uniform h)
+  const int* mrange;   // mass-uniformity pair (FP32 bits; equal: uniform m)
 };
 
 template <typename T>
@@ -228,6 +229,7 @@ inline View<T> to_view(const sph_view& a) {
   v.pos3 = (const float*)a.pos3;
   v.pm4 = (const float4*)a.pm4;
   v.hrange = (const int*)a.hrange;
+  v.mrange = (const int*)a.mrange;
   return v;
 }
 

@@ -65,6 +65,11 @@ constexpr int kDensUnroll = SPH_DENS_UNROLL;  // candidates per screen step, den
 #define SPH_DENS_DIRECT 1
 #endif
 constexpr bool kDensDirect = SPH_DENS_DIRECT != 0;
+// queue-free density sources of uniform mass are read from the (x,y,z) copy
+#ifndef SPH_DENS_XYZ
+#define SPH_DENS_XYZ 1
+#endif
+constexpr bool kDensXyz = SPH_DENS_XYZ != 0;
 // a density pass over two sources of uniform h sums them row by row
 // together (one source, or more than two, take the source-by-source loop)
 constexpr int kDirectSrc = 2;
@@ -340,35 +345,63 @@ __device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buff
   }
 }
 
-// Summation density over the candidates j0 .. j0+U-1 (those below j1) of
-// one stencil row of a source whose h is uniform: no queue.  Every lane
-// loads U (x, y, z, m) records (pm4, through the bounds-checked buffer view
-// `rs`, one base offset and the slots as immediate offsets), screens them
-// like screen_block (FP32 test, FP64 inside the +-2e-6 band) and adds
-// m 4w(q) of every accepted one, q = |r_a - r_b| * inv from the screened
-// squared distance, in row order (the order a queue would drain them in).
-// 4w is four times the cubic spline's shape (kernels.pyx:367-374):
-// (2 - q)^3 for 1 < q < 2, 4 - 6 q^2 + 3 q^3 for q <= 1; the caller scales
-// the row sums by fac / 4 (the normalisation of a source of uniform h is
-// the same for every pair), which saves the per-pair multiplications by the
-// constants (the pass is bound by VALU issue).  Evaluating w for every
-// candidate costs about what queueing the accepted ones costs, and it
-// removes the drain loop with its dependent index read, gather and memory
-// wait per pair.  FP32 pair arithmetic only.
-template <int U>
-__device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int j0, int j1,
+// 4^(1/3): with v = 4^(1/3) max(1 - q, 0), v^3 = 4 max(1 - q, 0)^3.
+constexpr float kCbrt4 = 1.5874010519681994f;
+
+// Summation density over U candidate slots of one stencil row of a source
+// whose h is uniform: no queue.  `off` is the byte offset of the first slot
+// in the source's candidate records, loaded through the bounds-checked
+// buffer view `rs` (one base offset, the slots as immediate offsets), and
+// `rem` the bytes left in the lane's range, so slot u is a candidate when
+// u * stride < rem.  With P3 the records are the 12-byte (x, y, z) copy
+// (pos3, four slots in three 16-byte loads) and every source particle has
+// the mass `mb` (the source's mass is uniform); otherwise they are the
+// 16-byte (x, y, z, m) records of pm4.  Both give the same products, so the
+// same bits; the 12-byte records move a quarter fewer bytes through the
+// vector memory pipeline, which the pass keeps busy.  Every lane screens its
+// slots like screen_block (FP32 test, FP64 inside the +-2e-6 band) and adds
+// m 4w(q) of every accepted one, in row order (the order a queue would
+// drain them in).  4w is four times the cubic spline's shape
+// (kernels.pyx:367-374), 4 - 6 q^2 + 3 q^3 for q <= 1 and (2 - q)^3 for
+// 1 < q < 2, written branch-free as t^3 - v^3 with t = max(2 - q, 0) and
+// v = 4^(1/3) max(1 - q, 0) (t^3 - 4 (1 - q)^3 is the inner polynomial),
+// q = |r_a - r_b| / h_ab from the screened squared distance; `ninv` =
+// -1/h_ab and `nkinv` = -4^(1/3)/h_ab.  The caller scales the row sums by
+// fac / 4 (the normalisation of a source of uniform h is the same for every
+// pair), which saves the per-pair multiplications by the constants.
+// Evaluating w for every slot costs about what queueing the accepted ones
+// costs, and it removes the drain loop with its dependent index read,
+// gather and memory wait per pair.  FP32 pair arithmetic only.
+template <int U, bool P3>
+__device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int off, int rem,
                                                      const float4& pd, float r2lo, float r2hi,
-                                                     double r2d, float inv, float& nr) {
+                                                     double r2d, float ninv, float nkinv,
+                                                     float mb, float& nr) {
+  static_assert(!P3 || U % 4 == 0, "pos3 blocks hold whole groups of four slots");
+  constexpr int kStride = P3 ? 12 : 16;
   float4 c[U];
-  int off = (int)((unsigned)j0 * 16u);
-  asm("" : "+v"(off));
+  if constexpr (P3) {
 #pragma unroll
-  for (int u = 0; u < U; ++u) {
-    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * u, 0, 0);
-    c[u] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
-                       __uint_as_float(v[3]));
+    for (int g = 0; g < U / 4; ++g) {
+      float w[12];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 48 * g + 16 * k, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[4 * k + e] = __uint_as_float(v[e]);
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        c[4 * g + u] = make_float4(w[3 * u], w[3 * u + 1], w[3 * u + 2], mb);
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off + 16 * u, 0, 0);
+      c[u] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                         __uint_as_float(v[3]));
+    }
   }
-  const int rem = j1 - j0;
   bool ok[U], bnd[U];
   float d2[U];
   bool band = false;
@@ -376,10 +409,12 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
   for (int u = 0; u < U; ++u) {
     const float dx = c[u].x - pd.x, dy = c[u].y - pd.y, dz = c[u].z - pd.z;
     d2[u] = dx * dx + dy * dy + dz * dz;
-    const bool in = u < rem;
+    const bool in = kStride * u < rem;
     const bool lo = d2[u] < r2lo;
+    const bool hi = d2[u] <= r2hi;
     ok[u] = in && lo;
-    bnd[u] = in && !lo && d2[u] <= r2hi;
+    // lo implies hi (r2lo < r2hi); a NaN distance is neither
+    bnd[u] = in && (hi != lo);
     band = band || bnd[u];
   }
   if (__any(band)) {
@@ -391,13 +426,59 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
   }
 #pragma unroll
   for (int u = 0; u < U; ++u) {
-    const float q = __builtin_amdgcn_sqrtf(d2[u]) * inv;
-    // t = max(2 - q, 0) makes the outer branch exactly 0 from q = 2 on
-    const float t = fmaxf(2.f - q, 0.f);
-    const float outer = t * t * t;
-    const float inner = fmaf(q * q, fmaf(3.f, q, -6.f), 4.f);
-    const float w4 = q > 1.f ? outer : inner;
-    if (ok[u]) nr = fmaf(c[u].w, w4, nr);
+    const float s = __builtin_amdgcn_sqrtf(d2[u]);
+    // t = max(2 - q, 0) makes w exactly 0 from q = 2 on
+    const float t = fmaxf(fmaf(ninv, s, 2.f), 0.f);
+    const float v = fmaxf(fmaf(nkinv, s, kCbrt4), 0.f);
+    const float w4 = fmaf(-(v * v), v, t * t * t);
+    nr = fmaf(ok[u] ? c[u].w : 0.f, w4, nr);
+  }
+}
+
+// The candidate records of a density source of uniform h for
+// density_direct_block: the (x, y, z) copy when the source also has a
+// uniform mass (returned in mb), else the (x, y, z, m) records.
+struct DirectSrc {
+  __amdgpu_buffer_rsrc_t rs;
+  float mb;
+  bool p3;
+};
+
+__device__ __forceinline__ DirectSrc direct_src(const View<float>& S) {
+  DirectSrc d;
+  d.p3 = kDensXyz && S.pos3 != nullptr && S.mrange != nullptr && S.mrange[0] == S.mrange[1];
+  d.mb = d.p3 ? __int_as_float(S.mrange[0]) : 0.f;
+  d.rs = d.p3 ? __builtin_amdgcn_make_buffer_rsrc((void*)S.pos3, (short)0,
+                                                  (int)((unsigned)S.n * 12u), 0x00020000)
+              : __builtin_amdgcn_make_buffer_rsrc((void*)S.pm4, (short)0,
+                                                  (int)((unsigned)S.n * 16u), 0x00020000);
+  return d;
+}
+
+// Every candidate block of one stencil row [j0, j1) of a direct source.
+__device__ __forceinline__ void density_direct_row(const DirectSrc& d, int j0, int j1,
+                                                   const float4& pd, float r2lo, float r2hi,
+                                                   double r2d, float inv, float& nr) {
+  // byte offsets; off is kept opaque, so that the slots become immediate
+  // load offsets (an empty register constraint: no instruction is emitted)
+  if (d.p3) {
+    int off = (int)((unsigned)j0 * 12u);
+    const int end = (int)((unsigned)j1 * 12u);
+    asm("" : "+v"(off));
+    do {
+      density_direct_block<kDensUnroll, true>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
+                                              -kCbrt4 * inv, d.mb, nr);
+      off += 12 * kDensUnroll;
+    } while (__any(off < end));
+  } else {
+    int off = (int)((unsigned)j0 * 16u);
+    const int end = (int)((unsigned)j1 * 16u);
+    asm("" : "+v"(off));
+    do {
+      density_direct_block<kDensUnroll, false>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
+                                               -kCbrt4 * inv, 0.f, nr);
+      off += 16 * kDensUnroll;
+    } while (__any(off < end));
   }
 }
 
@@ -501,8 +582,10 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
     if (all_uni) {
       float inv[kDirectSrc], fac[kDirectSrc], nrs[kDirectSrc];
       bool scan[kDirectSrc];
+      DirectSrc ds[kDirectSrc];
 #pragma unroll
       for (int s = 0; s < kDirectSrc; ++s) {
+        ds[s] = direct_src(a.src[s < a.nsrc ? s : 0]);
         nrs[s] = 0.f;
         scan[s] = s < a.nsrc && !(s == a.cache_src && a.cache_mode == 2);
         const float hb = s < a.nsrc ? __int_as_float(a.src[s].hrange[0]) : 1.f;
@@ -528,13 +611,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
               j1 = S.cstart[k1];
             }
             if (!__any(j0 < j1)) continue;
-            const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
-                (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
-            do {
-              density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv[s],
-                                                nrs[s]);
-              j0 += kDensUnroll;
-            } while (__any(j0 < j1));
+            density_direct_row(ds[s], j0, j1, pd, r2lo, r2hi, r2d, inv[s], nrs[s]);
           }
         }
       }
@@ -578,8 +655,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
       const float inv = frcp<float>(hab);
       const float fac = h_dim_inv<float>(inv, dim);
       float nrf = 0.f;
-      const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
-          (void*)S.pm4, (short)0, (int)((unsigned)S.n * 16u), 0x00020000);
+      const DirectSrc d = direct_src(S);
       for (int dz = -ring; dz <= ring; ++dz) {
         const bool zok = local && dz >= dzlo && dz <= dzhi;
         if (!__any(zok)) continue;
@@ -588,10 +664,7 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
           if (!__any(yok)) continue;
           int j0 = 0, j1 = 0;
           if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
-          while (__any(j0 < j1)) {
-            density_direct_block<kDensUnroll>(rp, j0, j1, pd, r2lo, r2hi, r2d, inv, nrf);
-            j0 += kDensUnroll;
-          }
+          if (__any(j0 < j1)) density_direct_row(d, j0, j1, pd, r2lo, r2hi, r2d, inv, nrf);
         }
       }
       nr = (CT)(nrf * (0.25f * fac));

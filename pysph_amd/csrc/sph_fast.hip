@@ -201,7 +201,8 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __r
                                                       float4* __restrict__ vel4,
                                                       float* __restrict__ pos3,
                                                       float4* __restrict__ pm4,
-                                                      int* __restrict__ hrange) {
+                                                      int* __restrict__ hrange,
+                                                      int* __restrict__ mrange) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = i < a.n;
   if (valid) {
@@ -222,11 +223,17 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __r
     const int hb = __float_as_int(a.h[i]);
     if (hb != __float_as_int(a.h[0])) hrange[1] = hb;
   }
+  // the same for the mass (mrange)
+  if (mrange != nullptr && valid) {
+    const int mb = __float_as_int(a.m[i]);
+    if (mb != __float_as_int(a.m[0])) mrange[1] = mb;
+  }
 }
 
-__global__ void hrange_init(const float* __restrict__ h, int* __restrict__ hrange) {
-  hrange[0] = __float_as_int(h[0]);
-  hrange[1] = __float_as_int(h[0]);
+// (v_0, v_0) into a uniformity pair
+__global__ void range_init(const float* __restrict__ v, int* __restrict__ range) {
+  range[0] = __float_as_int(v[0]);
+  range[1] = __float_as_int(v[0]);
 }
 
 // Compile-time dimension of the FP32-pair scan kernels (0: from the
@@ -359,22 +366,26 @@ int32_t fast_momentum(hipStream_t st, const sph_grid& g, double rmax, PassArgs<f
 using namespace sph;
 
 extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, void* vel4_dev,
-                            void* pos3_dev, void* pm4_dev, int32_t* hrange_dev) {
+                            void* pos3_dev, void* pm4_dev, int32_t* hrange_dev,
+                            int32_t* mrange_dev) {
   SPH_TRY(check_view(a, "sph_pack", vel4_dev != nullptr, false, false));
   if (a->dtype != SPH_F32) {
     set_error("sph_pack: packed copies exist for FP32 storage only");
     return SPH_EUNSUPPORTED;
   }
-  if ((vel4_dev != nullptr || pm4_dev != nullptr) && a->n > 0 && !a->m) {
-    set_error("sph_pack: m is required for vel4 and pm4");
+  if ((vel4_dev != nullptr || pm4_dev != nullptr || mrange_dev != nullptr) && a->n > 0 &&
+      !a->m) {
+    set_error("sph_pack: m is required for vel4, pm4 and mrange");
     return SPH_EINVAL;
   }
   if (a->n == 0) return SPH_OK;
   hipStream_t st = as_stream(stream);
   if (hrange_dev != nullptr)
-    hipLaunchKernelGGL(hrange_init, dim3(1), dim3(1), 0, st, (const float*)a->h, hrange_dev);
+    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, st, (const float*)a->h, hrange_dev);
+  if (mrange_dev != nullptr)
+    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, st, (const float*)a->m, mrange_dev);
   hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, st,
                      to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev,
-                     (float4*)pm4_dev, (int*)hrange_dev);
+                     (float4*)pm4_dev, (int*)hrange_dev, (int*)mrange_dev);
   return launch_status("sph_pack");
 }

@@ -285,6 +285,7 @@ class WCSPHPlan:
             pk = {"n": n, "pos4": mk(), "vel4": mk(), "aux4": mk(), "pm4": mk(),
                   "pos3": torch.empty(max(n, 1) * 3, dtype=torch.float32, device=pa.device),
                   "hrange": torch.empty(2, dtype=torch.int32, device=pa.device),
+                  "mrange": torch.empty(2, dtype=torch.int32, device=pa.device),
                   "version": None}
             self._pk[id(pa)] = pk
         return pk
@@ -300,11 +301,12 @@ class WCSPHPlan:
         v = pa.view()
         for f, t in (fields or {}).items():
             setattr(v, f, ptr(t))
-        # the stage-2 repack (fields: X1 positions and velocities) keeps h,
-        # so the h-uniformity pair of the step's first pack stays valid
+        # the stage-2 repack (fields: X1 positions and velocities) keeps h
+        # and m, so the uniformity pairs of the step's first pack stay valid
         hr = None if fields else pk["hrange"].data_ptr()
+        mr = None if fields else pk["mrange"].data_ptr()
         _abi.call("sph_pack", stream, v, pk["pos4"].data_ptr(), pk["vel4"].data_ptr(),
-                  pk["pos3"].data_ptr(), pk["pm4"].data_ptr(), hr)
+                  pk["pos3"].data_ptr(), pk["pm4"].data_ptr(), hr, mr)
         pk["version"] = ver
 
     def _pack_all(self):
@@ -319,7 +321,8 @@ class WCSPHPlan:
             for pa in self.particles.arrays:
                 pk = self._pk[id(pa)]
                 ov[id(pa)] = {"pos4": pk["pos4"], "vel4": pk["vel4"], "aux4": pk["aux4"],
-                              "pos3": pk["pos3"], "pm4": pk["pm4"], "hrange": pk["hrange"]}
+                              "pos3": pk["pos3"], "pm4": pk["pm4"], "hrange": pk["hrange"],
+                              "mrange": pk["mrange"]}
         if s1 is not None:
             ov.setdefault(id(self.fluid), {}).update(s1)
         return ov

@@ -93,6 +93,7 @@ def packed_override(arrays, override=None):
             sc["pos3"] = torch.empty(max(n, 1) * 3, dtype=torch.float32, device=pa.device)
             sc["pm4"] = torch.empty((max(n, 1), 4), dtype=torch.float32, device=pa.device)
             sc["hrange"] = torch.empty(2, dtype=torch.int32, device=pa.device)
+            sc["mrange"] = torch.empty(2, dtype=torch.int32, device=pa.device)
             pa.__dict__["_pack_scratch"] = sc
         # pack what the pass would read: the array's fields as overridden
         v = pa.view()
@@ -100,7 +101,7 @@ def packed_override(arrays, override=None):
             setattr(v, f, ptr(t))
         _abi.call("sph_pack", require_gpu(sc["pos4"]), v, sc["pos4"].data_ptr(),
                   sc["vel4"].data_ptr(), sc["pos3"].data_ptr(), sc["pm4"].data_ptr(),
-                  sc["hrange"].data_ptr())
+                  sc["hrange"].data_ptr(), sc["mrange"].data_ptr())
         d = dict(ov.get(id(pa), {}))
         d.update(sc)
         ov[id(pa)] = d

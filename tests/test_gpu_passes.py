@@ -411,6 +411,56 @@ def test_eval_accumulates_into_local_and_zeroes_the_rest(precision):
         assert np.max(np.abs(got[local] - want)) < TOL[precision] * scale * 10, nm
 
 
+def test_density_xyz_records_with_uniform_mass():
+    """A density source of uniform h and uniform mass is read from the
+    12-byte (x, y, z) copy, its mass taken from the pack's mass-uniformity
+    pair: the same bits as the 16-byte (x, y, z, m) records (pair withheld),
+    with one and two sources and in the fused launch.  A source whose masses
+    differ keeps the 16-byte records (its pair is unequal) and matches the
+    oracle."""
+    f, b, info = gen.dam_break_3d(dx=1.0 / 40, column=(10, 10, 20), tank=(24, 10, 20), seed=7,
+                                  precision="single")
+    particles = base.Particles([f, b])
+    cm = particles.cell_manager
+    cm.set_cells_per_radius(2)
+    cm.update()
+    prm = _ops.make_params(3, co=info["co"], ro=info["rho0"], gamma=7.0)
+    ov = _ops.packed_override([f, b])
+    for pa in (f, b):
+        mr = ov[id(pa)]["mrange"].cpu().numpy()
+        assert mr[0] == mr[1] == np.float32(pa.get("m")[0]).view(np.int32), pa.name
+    ov16 = {k: dict(v, mrange=None) for k, v in ov.items()}
+
+    def rho_of(dst, srcs, o):
+        r = torch.full_like(dst.get_carray("rho"), -1.0)
+        _ops.density(cm, prm, dst, srcs, r, override=o)
+        torch.cuda.synchronize()
+        return r.cpu().numpy()
+
+    for dst, srcs in ((f, [f]), (f, [f, b]), (b, [f, b])):
+        a, c = rho_of(dst, srcs, ov), rho_of(dst, srcs, ov16)
+        assert np.array_equal(a, c), (dst.name, len(srcs))
+    outs = []
+    for o in (ov, ov16):
+        res = [torch.full_like(pa.get_carray("rho"), -1.0) for pa in (f, b)]
+        _ops.density_multi(cm, prm, [f, b], [f, b], res, [None, None], [None, None], override=o)
+        torch.cuda.synchronize()
+        outs.append([t.cpu().numpy() for t in res])
+    for a, c in zip(*outs):
+        assert np.array_equal(a, c)
+    # masses that differ: the pair is unequal, the 16-byte records are read
+    n = f.get_number_of_particles()
+    rng = np.random.Generator(np.random.PCG64(5))
+    f.set(m=f.get("m") * (1.0 + 0.2 * rng.random(n)))
+    ov2 = _ops.packed_override([f, b])
+    mr = ov2[id(f)]["mrange"].cpu().numpy()
+    assert mr[0] != mr[1]
+    got = np.empty(n)
+    got[f._order_np()] = rho_of(f, [f, b], ov2)
+    rf, rb = to_oracle(f), to_oracle(b)
+    assert max_rel(got, O.sphrho(3, rf, [rf, rb])) < 1e-5
+
+
 def test_density_multi_one_launch_equals_two():
     """sph_density_multi (the fluid's and the boundary's SPHRho + Tait in one
     launch) writes the same bits as two sph_density calls, and matches the
