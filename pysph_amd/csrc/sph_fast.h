@@ -354,7 +354,8 @@ constexpr float kCbrt4 = 1.5874010519681994f;
 // buffer view `rs` (one base offset, the slots as immediate offsets), and
 // `rem` the bytes left in the lane's range, so slot u is a candidate when
 // u * stride < rem.  With P3 the records are the 12-byte (x, y, z) copy
-// (pos3, four slots in three 16-byte loads) and every source particle has
+// (pos3: four slots in three 16-byte loads, two in a 16-byte and an 8-byte
+// load) and every source particle has
 // the mass `mb` (the source's mass is uniform); otherwise they are the
 // 16-byte (x, y, z, m) records of pm4.  Both give the same products, so the
 // same bits; the 12-byte records move a quarter fewer bytes through the
@@ -377,10 +378,16 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
                                                      const float4& pd, float r2lo, float r2hi,
                                                      double r2d, float ninv, float nkinv,
                                                      float mb, float& nr) {
-  static_assert(!P3 || U % 4 == 0, "pos3 blocks hold whole groups of four slots");
+  static_assert(!P3 || U % 4 == 0 || U == 2, "pos3 blocks: groups of four slots, or two");
   constexpr int kStride = P3 ? 12 : 16;
   float4 c[U];
-  if constexpr (P3) {
+  if constexpr (P3 && U == 2) {
+    // two slots: 24 bytes, one 16-byte and one 8-byte load
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
+    const auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, off + 16, 0, 0);
+    c[0] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), mb);
+    c[1] = make_float4(__uint_as_float(v[3]), __uint_as_float(t[0]), __uint_as_float(t[1]), mb);
+  } else if constexpr (P3) {
 #pragma unroll
     for (int g = 0; g < U / 4; ++g) {
       float w[12];
@@ -456,6 +463,12 @@ __device__ __forceinline__ DirectSrc direct_src(const View<float>& S) {
 }
 
 // Every candidate block of one stencil row [j0, j1) of a direct source.
+// From the (x, y, z) copy: blocks of four slots while some lane of the wave
+// has four candidates left, then blocks of two, so that fewer slots past
+// the lanes' row ends are loaded and screened (the wave runs each block for
+// all its lanes; 2^24 cube: 8 % fewer slots).  From the (x, y, z, m)
+// records: blocks of four (the second block shape costs that kernel more
+// registers than it saves).
 __device__ __forceinline__ void density_direct_row(const DirectSrc& d, int j0, int j1,
                                                    const float4& pd, float r2lo, float r2hi,
                                                    double r2d, float inv, float& nr) {
@@ -465,19 +478,24 @@ __device__ __forceinline__ void density_direct_row(const DirectSrc& d, int j0, i
     int off = (int)((unsigned)j0 * 12u);
     const int end = (int)((unsigned)j1 * 12u);
     asm("" : "+v"(off));
-    do {
-      density_direct_block<kDensUnroll, true>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
-                                              -kCbrt4 * inv, d.mb, nr);
-      off += 12 * kDensUnroll;
-    } while (__any(off < end));
+    while (__any(end - off >= 48)) {
+      density_direct_block<4, true>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
+                                    -kCbrt4 * inv, d.mb, nr);
+      off += 48;
+    }
+    while (__any(off < end)) {
+      density_direct_block<2, true>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
+                                    -kCbrt4 * inv, d.mb, nr);
+      off += 24;
+    }
   } else {
     int off = (int)((unsigned)j0 * 16u);
     const int end = (int)((unsigned)j1 * 16u);
     asm("" : "+v"(off));
     do {
-      density_direct_block<kDensUnroll, false>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
-                                               -kCbrt4 * inv, 0.f, nr);
-      off += 16 * kDensUnroll;
+      density_direct_block<4, false>(d.rs, off, end - off, pd, r2lo, r2hi, r2d, -inv,
+                                     -kCbrt4 * inv, 0.f, nr);
+      off += 64;
     } while (__any(off < end));
   }
 }
