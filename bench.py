@@ -1033,7 +1033,12 @@ def main():
             "valu_issue_busy": (valu_issue_busy(counters[passes[0][0]]["valu_per_wave"],
                                                 counters[passes[0][0]]["waves"],
                                                 ms_step / launches)
-                                if counters and passes[0][0] in counters else None)})
+                                if counters and passes[0][0] in counters else None),
+            # the texture-data unit (vector memory loads returned to the
+            # lanes): busy cycles per CU over the launch's cycles, from the
+            # committed counter summary
+            "td_busy": (counters[passes[0][0]].get("td_busy")
+                        if counters and passes[0][0] in counters else None)})
     kernels.sort(key=lambda k: -k["ms_per_step"])
     dom = kernels[0]
     out = {
@@ -1079,8 +1084,11 @@ def main():
                      "traffic_source": traffic_src,
                      "bytes_per_launch": dom["bytes_per_launch"],
                      "avg_launch_ms": dom["avg_launch_ms"],
-                     # what bounds these kernels: VALU issue (counters of
-                     # the committed SQ summary x this run's launch time)
+                     # what bounds these kernels: the vector memory data
+                     # path (TD busy, from the committed counter summary)
+                     # with VALU issue close behind (its instruction count
+                     # x 4 cycles over this run's launch time)
+                     "td_busy": dom["td_busy"],
                      "valu_issue_busy": dom["valu_issue_busy"],
                      "valu_issue_source": counters_src,
                      "launches_per_step": dom["launches_per_step"], "launches_timed": 20},
