@@ -64,6 +64,9 @@ def parse():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counters) and exit")
     ap.add_argument("--reuse", type=float, default=None, metavar="SKIN",
                     help="reuse stage-1 neighbour lists in stage 2 with this skin")
+    ap.add_argument("--lds-rows", action="store_true",
+                    help="stage the density scan's stencil rows in LDS (opt-in kernel variant: "
+                         "the same sums)")
     ap.add_argument("--cache-static", action="store_true",
                     help="keep the boundary-from-boundary density sums between calls "
                          "(same bits; those pairs are then evaluated once, not per stage)")
@@ -507,7 +510,7 @@ def main_cube(args):
     wall = time.perf_counter() - t0
     # the density launch alone, 20 back-to-back (idempotent: rho from positions)
     from pysph_amd.sph import _ops
-    prm = _ops.make_params(3, radius_scale=2.0)
+    prm = _ops.make_params(3, radius_scale=2.0, lds_rows=args.lds_rows)
     ov = _ops.packed_override([pa])
     rho = pa.get_carray("rho")
 
@@ -541,7 +544,7 @@ def main_cube(args):
         "config": {"workload": "unit-cube neighbour search + summation density, %s" % kind,
                    "n": n, "h": float(pa.get_carray("h")[0].item()),
                    "pairs_per_step": pairs, "mean_neighbours": pairs / n,
-                   "cells_per_radius": args.cells_per_radius,
+                   "cells_per_radius": args.cells_per_radius, "lds_rows": args.lds_rows,
                    "step": "re-bin (fixed domain) + pack + density", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "sph::density_scan<float, false, false>",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -957,6 +960,8 @@ def main():
     if args.cells_per_radius is None:
         args.cells_per_radius = s.integrator.plan.cells_per_radius
     s.integrator.plan.use_graph = not args.no_graph
+    if args.lds_rows:
+        s.integrator.plan.set_lds_rows(True)
     nf, nb = f.get_number_of_particles(), b.get_number_of_particles()
     assert s.integrator.plan is not None, "fused plan not selected"
     p_rho0, p_m0, p_rho_fluid0 = pair_counts(particles, f, b)
@@ -1064,7 +1069,7 @@ def main():
                    "fluid_lattice": list(col), "tank_cells": list(tank),
                    "n_fluid": nf, "n_boundary": nb, "dx": info["dx"], "h": info["h"],
                    "dt": info["dt"], "c0": info["co"], "pairs_per_step": pairs_per_step,
-                   "cache_static": args.cache_static,
+                   "cache_static": args.cache_static, "lds_rows": args.lds_rows,
                    # with the static cache the boundary-from-boundary density
                    # pairs are evaluated once, not in every stage
                    "pairs_evaluated_per_step": pairs_per_step - (

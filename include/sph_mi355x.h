@@ -147,6 +147,13 @@ typedef struct sph_params {
      * the self pair contributes 0.  The density then runs its SoA kernels;
      * the batched engine rejects it. */
     int32_t exclude_self;
+    /* 1: the queue-free FP32 density scan (sources of uniform h) stages each
+     * stencil row in LDS: the wave loads the union of its lanes' candidate
+     * ranges with coalesced 16-byte loads and every lane screens its own
+     * range from LDS, the same sums as the default per-lane loads (rows
+     * whose union exceeds 256 records take those).  Opt-in: slower on
+     * MI355X at the sizes measured.  The batched engine ignores it. */
+    int32_t lds_rows;
 } sph_params;
 
 int32_t sph_abi_version(void);

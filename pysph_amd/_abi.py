@@ -52,7 +52,7 @@ class SphParams(ctypes.Structure):
                 ("eta", ctypes.c_double), ("eps", ctypes.c_double),
                 ("g", ctypes.c_double * 3), ("has_gravity", ctypes.c_int32),
                 ("has_xsph", ctypes.c_int32), ("symmetric", ctypes.c_int32),
-                ("exclude_self", ctypes.c_int32)]
+                ("exclude_self", ctypes.c_int32), ("lds_rows", ctypes.c_int32)]
 
 
 class SphNlist(ctypes.Structure):

@@ -94,7 +94,7 @@ template <int DIM>
 __global__ __launch_bounds__(kBlock) void batch_density_kernel(
     const SceneArgs* __restrict__ args, int phase, int set, const Tile* __restrict__ tiles,
     int ntiles, const int* __restrict__ seg, int mode, int grab, int* counters) {
-  __shared__ int qbuf[kBlock / 64][kDensQueue][64];
+  __shared__ __align__(16) int qbuf[kBlock / 64][kDensQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   int* q = &qbuf[wid][0][lane];
   for_each_tile(mode, ntiles, seg, grab, counters, [&](int t) {

@@ -373,15 +373,24 @@ constexpr float kCbrt4 = 1.5874010519681994f;
 // Evaluating w for every slot costs about what queueing the accepted ones
 // costs, and it removes the drain loop with its dependent index read,
 // gather and memory wait per pair.  FP32 pair arithmetic only.
-template <int U, bool P3>
+// records a wave stages for density_lds_row: its 4 KB LDS region (the
+// queue area the queue-free density path does not use)
+constexpr int kStage = 256;
+
+template <int U, bool P3, bool FROM_LDS = false>
 __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int off, int rem,
                                                      const float4& pd, float r2lo, float r2hi,
                                                      double r2d, float ninv, float nkinv,
-                                                     float mb, float& nr) {
+                                                     float mb, float& nr,
+                                                     const float4* stage = nullptr) {
   static_assert(!P3 || U % 4 == 0 || U == 2, "pos3 blocks: groups of four slots, or two");
-  constexpr int kStride = P3 ? 12 : 16;
+  // with FROM_LDS, off and rem count records of the wave's staged row
+  constexpr int kStride = FROM_LDS ? 1 : (P3 ? 12 : 16);
   float4 c[U];
-  if constexpr (P3 && U == 2) {
+  if constexpr (FROM_LDS) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) c[u] = stage[(off + u) & (kStage - 1)];
+  } else if constexpr (P3 && U == 2) {
     // two slots: 24 bytes, one 16-byte and one 8-byte load
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0);
     const auto t = __builtin_amdgcn_raw_buffer_load_b64(rs, off + 16, 0, 0);
@@ -500,6 +509,71 @@ __device__ __forceinline__ void density_direct_row(const DirectSrc& d, int j0, i
   }
 }
 
+// Minimum / maximum over the wave (DPP row shifts within rows of 16 lanes,
+// then row broadcasts; lane 63 ends with the result).
+__device__ __forceinline__ int wave_min_dpp(int v) {
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x111, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x112, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x114, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x118, 0xf, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x142, 0xa, 0xf, false));
+  v = min(v, __builtin_amdgcn_update_dpp(INT_MAX, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+__device__ __forceinline__ int wave_max_dpp(int v) {
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x111, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x112, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x114, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x118, 0xf, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x142, 0xa, 0xf, false));
+  v = max(v, __builtin_amdgcn_update_dpp(INT_MIN, v, 0x143, 0xc, 0xf, false));
+  return __builtin_amdgcn_readlane(v, 63);
+}
+
+static_assert(kStage * 16 == kDensQueue * 64 * 4, "one wave's LDS region");
+
+// One stencil row of a direct source through LDS: the wave loads the union
+// of its lanes' ranges [lo, hi) of (x, y, z, m) records into its region with
+// coalesced 16-byte loads, then every lane screens its own range from LDS
+// in the same slot order as density_direct_row (the same sums).  False,
+// with nothing done, when the union exceeds the region or a lane's range is
+// not inside it.
+__device__ __forceinline__ bool density_lds_row(const DirectSrc& d, const float4* __restrict__ pm4,
+                                                float4* stage, int j0, int j1, const float4& pd,
+                                                float r2lo, float r2hi, double r2d, float inv,
+                                                float& nr) {
+  const bool has = j0 < j1;
+  const int lo = wave_min_dpp(has ? j0 : INT_MAX);
+  const int hi = wave_max_dpp(has ? j1 : INT_MIN);
+  if (lo >= hi || hi - lo > kStage) return false;
+  const int nu = hi - lo;
+  if (!__all(!has || (j0 >= lo && j1 <= hi))) return false;
+  const int lane = threadIdx.x & 63;
+  for (int k = lane; k < nu; k += 64) stage[k] = pm4[lo + k];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int o = has ? j0 - lo : 0;
+  int rem = has ? j1 - j0 : 0;
+  while (__any(rem >= 4)) {
+    density_direct_block<4, false, true>(d.rs, o, rem, pd, r2lo, r2hi, r2d, -inv, -kCbrt4 * inv,
+                                         0.f, nr, stage);
+    o += 4;
+    rem -= 4;
+  }
+  while (__any(rem > 0)) {
+    density_direct_block<2, false, true>(d.rs, o, rem, pd, r2lo, r2hi, r2d, -inv, -kCbrt4 * inv,
+                                         0.f, nr, stage);
+    o += 2;
+    rem -= 2;
+  }
+  // every lane's reads are done before the region is written again
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  return true;
+}
+
 // Whether a list built with the given skin still holds every neighbour of a
 // destination of search radius r after a displacement of at most disp.
 __device__ __forceinline__ bool list_valid(const float* disp, float skin, double r) {
@@ -558,7 +632,7 @@ __device__ __forceinline__ void density_epilogue(const PassArgs<float>& a, int i
 // list (superset radius).
 // DIM > 0 fixes the dimension at compile time (the kernels of dimensions 2
 // and 3); DIM = 0 reads it from the arguments.
-template <typename CT, bool TAIT, bool BUILD, int DIM = 0>
+template <typename CT, bool TAIT, bool BUILD, int DIM = 0, bool LDS = false>
 __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<float>& a,
                                                   int i, int* q) {
   const bool valid = i < a.dst.n;
@@ -629,7 +703,9 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
               j1 = S.cstart[k1];
             }
             if (!__any(j0 < j1)) continue;
-            density_direct_row(ds[s], j0, j1, pd, r2lo, r2hi, r2d, inv[s], nrs[s]);
+            if (!(LDS && density_lds_row(ds[s], S.pm4, (float4*)(q - (threadIdx.x & 63)), j0,
+                                         j1, pd, r2lo, r2hi, r2d, inv[s], nrs[s])))
+              density_direct_row(ds[s], j0, j1, pd, r2lo, r2hi, r2d, inv[s], nrs[s]);
           }
         }
       }
@@ -682,7 +758,11 @@ __device__ __forceinline__ void density_scan_body(const Grid& g, const PassArgs<
           if (!__any(yok)) continue;
           int j0 = 0, j1 = 0;
           if (yok) row_range_cull(g, S.cstart, cx, cy + dy, cz + dz, ring, cu, j0, j1);
-          if (__any(j0 < j1)) density_direct_row(d, j0, j1, pd, r2lo, r2hi, r2d, inv, nrf);
+          if (__any(j0 < j1)) {
+            if (!(LDS && density_lds_row(d, S.pm4, (float4*)(q - (threadIdx.x & 63)), j0, j1,
+                                         pd, r2lo, r2hi, r2d, inv, nrf)))
+              density_direct_row(d, j0, j1, pd, r2lo, r2hi, r2d, inv, nrf);
+          }
         }
       }
       nr = (CT)(nrf * (0.25f * fac));

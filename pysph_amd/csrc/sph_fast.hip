@@ -11,35 +11,66 @@ __global__ __launch_bounds__(kBlock)
 __attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
 #endif
 void density_scan(Grid g, PassArgs<float> a) {
-  __shared__ int qbuf[kBlock / 64][kDensQueue][64];
+  __shared__ __align__(16) int qbuf[kBlock / 64][kDensQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);
   density_scan_body<CT, TAIT, BUILD, DIM>(g, a, bid * kBlock + threadIdx.x,
                                           &qbuf[wid][0][lane]);
 }
 
-// The density passes of two destination arrays over the same sources in
-// one launch: blocks [0, nb0) serve a0's destinations, the others a1's.
-// One grid fills the GPU while the longer pass drains, which two launches
-// would need a second stream for.
-template <typename CT, bool TAIT, int DIM>
+// density_scan (FP32 pairs, no list) with the stencil rows of sources of
+// uniform h staged in LDS (sph_params.lds_rows, opt-in).
+template <bool TAIT, int DIM>
 __global__ __launch_bounds__(kBlock)
 #if SPH_DENS_WAVES > 0
 __attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
 #endif
-void density_scan2(Grid g, PassArgs<float> a0, PassArgs<float> a1, int nb0) {
-  __shared__ int qbuf[kBlock / 64][kDensQueue][64];
+void density_scan_lds(Grid g, PassArgs<float> a) {
+  __shared__ __align__(16) int qbuf[kBlock / 64][kDensQueue][64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int bid = xcd_block(blockIdx.x, gridDim.x);
+  density_scan_body<float, TAIT, false, DIM, true>(g, a, bid * kBlock + threadIdx.x,
+                                                   &qbuf[wid][0][lane]);
+}
+
+// The density passes of two destination arrays over the same sources in
+// one launch: blocks [0, nb0) serve a0's destinations, the others a1's.
+// One grid fills the GPU while the longer pass drains, which two launches
+// would need a second stream for.
+template <typename CT, bool TAIT, int DIM, bool LDS>
+__device__ __forceinline__ void density_scan2_run(const Grid& g, const PassArgs<float>& a0,
+                                                  const PassArgs<float>& a1, int nb0) {
+  __shared__ __align__(16) int qbuf[kBlock / 64][kDensQueue][64];
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const int bid = xcd_block(blockIdx.x, gridDim.x);
   // a block-uniform branch with one body per argument set: selecting a
   // reference to either kernel argument at run time would copy them to
   // scratch memory and read every field from there
   if (bid < nb0)
-    density_scan_body<CT, TAIT, false, DIM>(g, a0, bid * kBlock + threadIdx.x,
-                                            &qbuf[wid][0][lane]);
+    density_scan_body<CT, TAIT, false, DIM, LDS>(g, a0, bid * kBlock + threadIdx.x,
+                                                 &qbuf[wid][0][lane]);
   else
-    density_scan_body<CT, TAIT, false, DIM>(g, a1, (bid - nb0) * kBlock + threadIdx.x,
-                                            &qbuf[wid][0][lane]);
+    density_scan_body<CT, TAIT, false, DIM, LDS>(g, a1, (bid - nb0) * kBlock + threadIdx.x,
+                                                 &qbuf[wid][0][lane]);
+}
+
+template <typename CT, bool TAIT, int DIM>
+__global__ __launch_bounds__(kBlock)
+#if SPH_DENS_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
+#endif
+void density_scan2(Grid g, PassArgs<float> a0, PassArgs<float> a1, int nb0) {
+  density_scan2_run<CT, TAIT, DIM, false>(g, a0, a1, nb0);
+}
+
+// density_scan2 with the stencil rows staged in LDS (opt-in).
+template <bool TAIT, int DIM>
+__global__ __launch_bounds__(kBlock)
+#if SPH_DENS_WAVES > 0
+__attribute__((amdgpu_waves_per_eu(SPH_DENS_WAVES)))
+#endif
+void density_scan2_lds(Grid g, PassArgs<float> a0, PassArgs<float> a1, int nb0) {
+  density_scan2_run<float, TAIT, DIM, true>(g, a0, a1, nb0);
 }
 
 // Density from a list (exact test on every entry); lanes whose list is
@@ -250,6 +281,10 @@ static void density_scan_launch(hipStream_t st, int nb, const Grid& G, const Pas
   if (build) {
     if (tait) SPH_DENS((density_scan<CT, true, true, DIM>));
     else SPH_DENS((density_scan<CT, false, true, DIM>));
+  } else if (sizeof(CT) == 4 && a.lds_rows) {
+    // stencil rows staged in LDS (opt-in; FP32 pairs)
+    if (tait) SPH_DENS((density_scan_lds<true, DIM>));
+    else SPH_DENS((density_scan_lds<false, DIM>));
   } else {
     if (tait) SPH_DENS((density_scan<CT, true, false, DIM>));
     else SPH_DENS((density_scan<CT, false, false, DIM>));
@@ -297,12 +332,21 @@ static void density_pair_launch(hipStream_t st, const Grid& G, const PassArgs<fl
                                 const PassArgs<float>& a1, bool tait) {
   const int nb0 = nblocks_for(a0.dst.n), nb1 = nblocks_for(a1.dst.n);
   if (nb0 + nb1 == 0) return;
-  if (tait)
+  if (a0.lds_rows) {
+    // stencil rows staged in LDS (opt-in)
+    if (tait)
+      hipLaunchKernelGGL((density_scan2_lds<true, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0, st,
+                         G, a0, a1, nb0);
+    else
+      hipLaunchKernelGGL((density_scan2_lds<false, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0, st,
+                         G, a0, a1, nb0);
+  } else if (tait) {
     hipLaunchKernelGGL((density_scan2<float, true, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0, st,
                        G, a0, a1, nb0);
-  else
+  } else {
     hipLaunchKernelGGL((density_scan2<float, false, DIM>), dim3(nb0 + nb1), dim3(kBlock), 0,
                        st, G, a0, a1, nb0);
+  }
 }
 
 int32_t fast_density_pair(hipStream_t st, const sph_grid& g, double rmax, PassArgs<float> a0,

@@ -84,6 +84,7 @@ struct PassArgs {
   double* cache_sums;
   int cache_src;
   int cache_mode;
+  int lds_rows;  // density: stencil rows staged in LDS (sph_params.lds_rows)
 };
 
 // Scheme parameters of a pass from the ABI struct.
@@ -104,6 +105,7 @@ inline void set_params(PassArgs<T>& a, const sph_params* prm) {
   a.has_gravity = prm->has_gravity;
   a.symmetric = prm->symmetric != 0;
   a.excl_self = (unsigned)prm->exclude_self;
+  a.lds_rows = prm->lds_rows != 0;
 }
 
 // Distance to a periodic image source at (sx,sy,sz), FP64 without

@@ -160,6 +160,11 @@ class WCSPHPlan:
 
     DEFAULT_CELLS_PER_RADIUS = 2
 
+    def set_lds_rows(self, on=True):
+        """Stage the density scan's stencil rows in LDS (sph_params.lds_rows;
+        the same sums, slower on MI355X at the sizes measured)."""
+        self.prm.lds_rows = 1 if on else 0
+
     def set_cells_per_radius(self, k):
         """Bins of the shared CellManager's grid per reference cell (k = 1:
         the reference's cells)."""

@@ -20,10 +20,11 @@ DEFAULT_PAIR_PREC = _abi.SPH_PAIR_NATIVE
 
 def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, beta=1.0,
                 eta=0.1, eps=0.5, g=(0.0, 0.0, 0.0), has_gravity=False, has_xsph=False,
-                pair_prec=None, symmetric=False, exclude_self=0):
+                pair_prec=None, symmetric=False, exclude_self=0, lds_rows=False):
     """sph_params; `symmetric` selects the variable-h pair sets of
     VarHNbrParticleLocator (nnps.pyx:654-749); bit s of `exclude_self`
-    drops the self pair of source s (the destination) from the density."""
+    drops the self pair of source s (the destination) from the density;
+    `lds_rows` stages the density scan's stencil rows in LDS (opt-in)."""
     p = _abi.SphParams()
     p.dim = int(dim)
     p.pair_prec = DEFAULT_PAIR_PREC if pair_prec is None else int(pair_prec)
@@ -36,6 +37,7 @@ def make_params(dim, radius_scale=2.0, co=1.0, ro=1000.0, gamma=7.0, alpha=1.0, 
     p.has_xsph = 1 if has_xsph else 0
     p.symmetric = 1 if symmetric else 0
     p.exclude_self = int(exclude_self)
+    p.lds_rows = 1 if lds_rows else 0
     return p
 
 

@@ -461,6 +461,52 @@ def test_density_xyz_records_with_uniform_mass():
     assert max_rel(got, O.sphrho(3, rf, [rf, rb])) < 1e-5
 
 
+def test_density_lds_rows_same_bits():
+    """The opt-in LDS-staged stencil rows (sph_params.lds_rows: the wave
+    loads the union of its lanes' candidate ranges into LDS, every lane
+    screens its own range from there) give the same bits as the per-lane
+    loads: one and two sources, uniform and varied masses, the fused launch,
+    and RK2 steps of the bench's plan."""
+    f, b, info = gen.dam_break_3d(dx=1.0 / 40, column=(10, 10, 20), tank=(24, 10, 20), seed=7,
+                                  precision="single")
+    particles = base.Particles([f, b])
+    cm = particles.cell_manager
+    cm.set_cells_per_radius(2)
+    cm.update()
+
+    def run(lds, massvar=False):
+        prm = _ops.make_params(3, co=info["co"], ro=info["rho0"], gamma=7.0, lds_rows=lds)
+        ov = _ops.packed_override([f, b])
+        out = []
+        for dst, srcs in ((f, [f]), (f, [f, b]), (b, [f, b])):
+            r = torch.full_like(dst.get_carray("rho"), -1.0)
+            _ops.density(cm, prm, dst, srcs, r, override=ov)
+            out.append(r)
+        res = [torch.full_like(pa.get_carray("rho"), -1.0) for pa in (f, b)]
+        _ops.density_multi(cm, prm, [f, b], [f, b], res, [None, None], [None, None], override=ov)
+        torch.cuda.synchronize()
+        return [t.cpu().numpy() for t in out + res]
+
+    for a, c in zip(run(False), run(True)):
+        assert np.array_equal(a, c)
+    n = f.get_number_of_particles()
+    rng = np.random.Generator(np.random.PCG64(9))
+    f.set(m=f.get("m") * (1.0 + 0.2 * rng.random(n)))
+    for a, c in zip(run(False), run(True)):
+        assert np.array_equal(a, c)
+    import bench
+    outs = []
+    for lds in (False, True):
+        s, parts, fl, bd, inf, col, tank = bench.build_case(scale=0.2)
+        s.integrator.plan.set_lds_rows(lds)
+        for _ in range(3):
+            s.integrator.integrate(inf["dt"])
+        torch.cuda.synchronize()
+        outs.append({c: fl.get(c) for c in ("x", "y", "z", "u", "v", "w", "rho", "p")})
+    for c in outs[0]:
+        assert np.array_equal(outs[0][c], outs[1][c]), c
+
+
 def test_density_multi_one_launch_equals_two():
     """sph_density_multi (the fluid's and the boundary's SPHRho + Tait in one
     launch) writes the same bits as two sph_density calls, and matches the
