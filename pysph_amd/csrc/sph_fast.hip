@@ -261,10 +261,17 @@ __global__ __launch_bounds__(kBlock) void pack_kernel(View<float> a, float4* __r
   }
 }
 
-// (v_0, v_0) into a uniformity pair
-__global__ void range_init(const float* __restrict__ v, int* __restrict__ range) {
-  range[0] = __float_as_int(v[0]);
-  range[1] = __float_as_int(v[0]);
+// (h_0, h_0) and (m_0, m_0) into the uniformity pairs (either may be NULL)
+__global__ void range_init(const float* __restrict__ h, int* __restrict__ hrange,
+                           const float* __restrict__ m, int* __restrict__ mrange) {
+  if (hrange != nullptr) {
+    hrange[0] = __float_as_int(h[0]);
+    hrange[1] = __float_as_int(h[0]);
+  }
+  if (mrange != nullptr) {
+    mrange[0] = __float_as_int(m[0]);
+    mrange[1] = __float_as_int(m[0]);
+  }
 }
 
 // Compile-time dimension of the FP32-pair scan kernels (0: from the
@@ -424,10 +431,9 @@ extern "C" int32_t sph_pack(void* stream, const sph_view* a, void* pos4_dev, voi
   }
   if (a->n == 0) return SPH_OK;
   hipStream_t st = as_stream(stream);
-  if (hrange_dev != nullptr)
-    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, st, (const float*)a->h, hrange_dev);
-  if (mrange_dev != nullptr)
-    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, st, (const float*)a->m, mrange_dev);
+  if (hrange_dev != nullptr || mrange_dev != nullptr)
+    hipLaunchKernelGGL(range_init, dim3(1), dim3(1), 0, st, (const float*)a->h, hrange_dev,
+                       (const float*)a->m, mrange_dev);
   hipLaunchKernelGGL(pack_kernel, dim3(nblocks_for(a->n)), dim3(kBlock), 0, st,
                      to_view<float>(*a), (float4*)pos4_dev, (float4*)vel4_dev, (float*)pos3_dev,
                      (float4*)pm4_dev, (int*)hrange_dev, (int*)mrange_dev);
