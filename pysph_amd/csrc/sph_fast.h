@@ -348,6 +348,10 @@ __device__ __forceinline__ void screen_block(const View<float>& S, __amdgpu_buff
 // 4^(1/3): with v = 4^(1/3) max(1 - q, 0), v^3 = 4 max(1 - q, 0)^3.
 constexpr float kCbrt4 = 1.5874010519681994f;
 
+// records a wave stages for density_lds_row: its 4 KB LDS region (the
+// queue area the queue-free density path does not use)
+constexpr int kStage = 256;
+
 // Summation density over U candidate slots of one stencil row of a source
 // whose h is uniform: no queue.  `off` is the byte offset of the first slot
 // in the source's candidate records, loaded through the bounds-checked
@@ -355,11 +359,13 @@ constexpr float kCbrt4 = 1.5874010519681994f;
 // `rem` the bytes left in the lane's range, so slot u is a candidate when
 // u * stride < rem.  With P3 the records are the 12-byte (x, y, z) copy
 // (pos3: four slots in three 16-byte loads, two in a 16-byte and an 8-byte
-// load) and every source particle has
-// the mass `mb` (the source's mass is uniform); otherwise they are the
-// 16-byte (x, y, z, m) records of pm4.  Both give the same products, so the
-// same bits; the 12-byte records move a quarter fewer bytes through the
-// vector memory pipeline, which the pass keeps busy.  Every lane screens its
+// load) and every source particle has the mass `mb` (the source's mass is
+// uniform); otherwise they are the 16-byte (x, y, z, m) records of pm4.
+// Both give the same products, so the same bits; the 12-byte records move a
+// quarter fewer bytes through the vector memory pipeline, which the pass
+// keeps busy.  With FROM_LDS the records are (x, y, z, m) entries of the
+// wave's staged row `stage` (density_lds_row), `off` and `rem` in records
+// instead of bytes.  Every lane screens its
 // slots like screen_block (FP32 test, FP64 inside the +-2e-6 band) and adds
 // m 4w(q) of every accepted one, in row order (the order a queue would
 // drain them in).  4w is four times the cubic spline's shape
@@ -373,10 +379,6 @@ constexpr float kCbrt4 = 1.5874010519681994f;
 // Evaluating w for every slot costs about what queueing the accepted ones
 // costs, and it removes the drain loop with its dependent index read,
 // gather and memory wait per pair.  FP32 pair arithmetic only.
-// records a wave stages for density_lds_row: its 4 KB LDS region (the
-// queue area the queue-free density path does not use)
-constexpr int kStage = 256;
-
 template <int U, bool P3, bool FROM_LDS = false>
 __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, int off, int rem,
                                                      const float4& pd, float r2lo, float r2hi,
@@ -384,7 +386,6 @@ __device__ __forceinline__ void density_direct_block(__amdgpu_buffer_rsrc_t rs, 
                                                      float mb, float& nr,
                                                      const float4* stage = nullptr) {
   static_assert(!P3 || U % 4 == 0 || U == 2, "pos3 blocks: groups of four slots, or two");
-  // with FROM_LDS, off and rem count records of the wave's staged row
   constexpr int kStride = FROM_LDS ? 1 : (P3 ? 12 : 16);
   float4 c[U];
   if constexpr (FROM_LDS) {
@@ -538,10 +539,10 @@ static_assert(kStage * 16 == kDensQueue * 64 * 4, "one wave's LDS region");
 // in the same slot order as density_direct_row (the same sums).  False,
 // with nothing done, when the union exceeds the region or a lane's range is
 // not inside it.
-__device__ __forceinline__ bool density_lds_row(const DirectSrc& d, const float4* __restrict__ pm4,
-                                                float4* stage, int j0, int j1, const float4& pd,
-                                                float r2lo, float r2hi, double r2d, float inv,
-                                                float& nr) {
+__device__ __forceinline__ bool density_lds_row(const DirectSrc& d,
+                                                const float4* __restrict__ pm4, float4* stage,
+                                                int j0, int j1, const float4& pd, float r2lo,
+                                                float r2hi, double r2d, float inv, float& nr) {
   const bool has = j0 < j1;
   const int lo = wave_min_dpp(has ? j0 : INT_MAX);
   const int hi = wave_max_dpp(has ? j1 : INT_MIN);
