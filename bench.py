@@ -653,8 +653,12 @@ def main_batch(args):
         "phase_ms": {"density_fluid": dens_ms, "stage1": stage_ms},
     }
     if tune is not None:
+        # the plain grid (mode 0) against the best persistent schedules
+        # (mode 1 static segments, mode 2 per-XCD work counters)
         out["autotune"] = {ph: {"best": r[0], "ms": r[1],
-                                "default_ms": next(t for c, t in r[2] if c["mode"] == 0)}
+                                "default_ms": next(t for c, t in r[2] if c["mode"] == 0),
+                                "best_persistent": min(((c, t) for c, t in r[2] if c["mode"]),
+                                                       key=lambda ct: ct[1], default=None)}
                            for ph, r in tune.items()}
     eng.close()
     if not args.no_cpu:
