@@ -567,7 +567,10 @@ typedef struct sph_mlp_graph {
     int64_t nedges;
     int64_t n_fluid;      /* nodes with btype 0 (loss normalisation) */
     int32_t dim;          /* 2 or 3 */
-    int32_t reserved;
+    /* SPH_MLP_INFER: sph_mlp_forward for inference only; it does not store
+     * the activations sph_mlp_backward reads, which then returns SPH_EINVAL
+     * until a forward pass without the flag */
+    int32_t flags;
     const float *pos4;    /* (x, y, z, h) per node */
     const float *vel4;    /* (u, v, w, m) per node */
     const float *rho;     /* density per node */
@@ -576,6 +579,7 @@ typedef struct sph_mlp_graph {
     const int64_t *nbr;   /* neighbour node of each edge */
     double rho0, co, gamma, u_scale, g, rho_scale, alpha, eta;
 } sph_mlp_graph;
+#define SPH_MLP_INFER 1
 int64_t sph_mlp_param_count(void);
 int32_t sph_mlp_create(void **model);
 int32_t sph_mlp_destroy(void *model);

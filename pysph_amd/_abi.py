@@ -104,11 +104,12 @@ SPH_MLP_HIDDEN = 128
 SPH_MLP_NIN = 16
 SPH_MLP_NE = 16
 SPH_MLP_ROUNDS = 2
+SPH_MLP_INFER = 1
 
 
 class SphMlpGraph(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int64), ("nedges", ctypes.c_int64), ("n_fluid", ctypes.c_int64),
-                ("dim", ctypes.c_int32), ("reserved", ctypes.c_int32)] + [
+                ("dim", ctypes.c_int32), ("flags", ctypes.c_int32)] + [
         (f, _vp) for f in ("pos4", "vel4", "rho", "btype", "off", "nbr")] + [
         (f, ctypes.c_double) for f in ("rho0", "co", "gamma", "u_scale", "g", "rho_scale",
                                        "alpha", "eta")]

@@ -121,7 +121,7 @@ class SurrogateWCSPH:
         n = g.n
         if self._out is None or self._out.shape[0] < n:
             self._out = torch.empty((max(n, 1), 2), dtype=torch.float32, device=g.device)
-        out = self.model.forward(g, self._out)
+        out = self.model.forward(g, self._out, infer=True)
         pa = g.pa
         # the kernel kicks by dt * a_scale * a and drifts by dt
         a_scale = 9.81 * (0.5 if self.inst else 1.0)

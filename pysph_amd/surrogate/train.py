@@ -136,7 +136,7 @@ def evaluate(model, data):
         for st, tgt in d["samples"]:
             g.set_state(st)
             g.build()
-            y = g.slot_rows_to_user(model.forward(g)[:g.n])
+            y = g.slot_rows_to_user(model.forward(g, infer=True)[:g.n])
             nf = g.n_fluid
             e = (y[:nf] - tgt[:nf])
             num += float((e * e).sum().item())

@@ -6,6 +6,8 @@ bfloat16 where the GPU stores them so (outputs and every gradient tensor
 within 1e-2 normwise relative: only the FP32 accumulation order differs),
 and the plain FP64 oracle (outputs within 3e-2, gradients within 0.2: the
 error that BF16 operands cost, reported)."""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -89,6 +91,26 @@ def test_forward_matches_oracle():
     assert errb < 3e-3
     assert err < 3e-2
     assert np.abs(Y).max() > 1e-3
+
+
+def test_inference_forward_same_output_and_no_backward():
+    """A forward pass for inference (SPH_MLP_INFER: the activations of the
+    backward pass are not stored) gives the same bits as a training forward
+    pass, and a backward pass after it is refused until a training forward
+    pass runs again."""
+    from pysph_amd import _abi
+    g, _ = _graph()
+    model = EdgeMLP(params=_params(3))
+    y_train = model.forward(g)[:g.n].cpu().numpy()
+    y_infer = model.forward(g, infer=True)[:g.n].cpu().numpy()
+    assert np.array_equal(y_train, y_infer)
+    dout = torch.zeros((max(g.n, 1), 2), dtype=torch.float32, device=g.device)
+    with pytest.raises(_abi.SphError):
+        _abi.call("sph_mlp_backward", model.handle, torch.cuda.current_stream().cuda_stream,
+                  ctypes.byref(g.struct), dout.data_ptr())
+    tgt = torch.zeros_like(dout)
+    loss = model.loss_backward(g, tgt)
+    assert torch.isfinite(loss).all()
 
 
 def test_backward_matches_oracle():
