@@ -127,55 +127,105 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
   }
 }
 
-// The same product for op(A) = A, op(B) = B (the forward layers and the
-// input gradients): each lane loads its 8 k-values of A as one 16-byte load,
-// and the block stages its 64-column panel of B in LDS transposed (column
-// major, k contiguous) so that every B fragment is one 16-byte LDS read.
-// Requires lda % 8 == 0, K % 16 == 0 and K <= kMaxK.
-constexpr int kMaxK = 256;
-constexpr int kLdsK = kMaxK + 8;  // padded pitch (bf16 elements)
+// Epilogue of the A B^T product with its loads issued ahead: the bias value
+// of the lane's column once, and the residual and accumulated values of all
+// 16 output elements before the first store, so that their latencies overlap
+// (per element, each load would be waited for in turn).  Rows and columns
+// past the matrix read row / column 0 and are not stored.  The arithmetic is
+// that of `gemm_kernel`'s epilogue, in the same order.
+struct EpiIn {
+  int rows[16];  // output row of element r, -1 when outside the matrix
+  float bv, rv[16], cv[16];
+};
 
-__device__ __forceinline__ void epilogue(const GemmArgs& g, const f32x16& acc, int orow0,
-                                         int col, int lane, float* C) {
+__device__ __forceinline__ void epi_load(const GemmArgs& g, int orow0, int col, int lane,
+                                         EpiIn& in) {
+  const bool cok = col < g.N;
+  const int ccol = cok ? col : 0;
+  in.bv = 0.f;
+  if (g.bias) in.bv = g.bias[ccol];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int orow = orow0 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-    if (orow >= g.M || col >= g.N) continue;
-    float v = acc[r];
-    if (g.bias) v += g.bias[col];
-    if (g.relu) v = fmaxf(v, 0.f);
-    if (g.U) g.U[(int64_t)orow * g.ldu + col] = v;
-    if (g.res) v += g.res[(int64_t)orow * g.ldr + col];
-    if (C) {
-      float* p = C + (int64_t)orow * g.ldc + col;
-      if (g.accumulate) v += *p;
-      *p = v;
-    }
-    if (g.Cb) g.Cb[(int64_t)orow * g.ldcb + col] = (__bf16)v;
+    in.rows[r] = (cok && orow < g.M) ? orow : -1;
+  }
+  if (g.res) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      in.rv[r] = g.res[(int64_t)(in.rows[r] < 0 ? 0 : in.rows[r]) * g.ldr + ccol];
+  }
+  if (g.C && g.accumulate) {
+#pragma unroll
+    for (int r = 0; r < 16; ++r)
+      in.cv[r] = g.C[(int64_t)(in.rows[r] < 0 ? 0 : in.rows[r]) * g.ldc + ccol];
   }
 }
+
+__device__ __forceinline__ void epi_store(const GemmArgs& g, const f32x16& acc, int col,
+                                          const EpiIn& in) {
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float v = acc[r];
+    if (g.bias) v += in.bv;
+    if (g.relu) v = fmaxf(v, 0.f);
+    const bool ok = in.rows[r] >= 0;
+    if (g.U && ok) g.U[(int64_t)in.rows[r] * g.ldu + col] = v;
+    if (g.res) v += in.rv[r];
+    if (g.C) {
+      if (g.accumulate) v += in.cv[r];
+      if (ok) g.C[(int64_t)in.rows[r] * g.ldc + col] = v;
+    }
+    if (g.Cb && ok) g.Cb[(int64_t)in.rows[r] * g.ldcb + col] = (__bf16)v;
+  }
+}
+
+// k-steps whose fragments are loaded before their MFMAs
+constexpr int kGemmChunk = 8;
 
 // C = A B^T-stored: A [M x K] row-major, B given as B^T [N x K] row-major
 // (k contiguous for both), so every fragment is one 16-byte load from
 // global memory / L2 and no LDS staging is needed.  The forward layers use
 // transposed BF16 copies of the weights; the input gradients use the
 // weights themselves.  Requires lda, ldb % 8 == 0 and K % 16 == 0.
+// KS = K / 16 when the depth is one of the network's (1, 8 or 16 steps):
+// the fragments of kGemmChunk k-steps are loaded before their MFMAs, so
+// their latencies overlap; KS = 0 walks a run-time K one step at a time.
+// Rows (columns) past M (N) load row (column) 0 and their outputs are not
+// stored, so no load is predicated.  The MFMAs run in k order either way.
+template <int KS>
 __global__ __launch_bounds__(256) void gemm_nt_kernel(GemmArgs g) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int wm = w & 1, wn = w >> 1;
   const int row = blockIdx.y * 64 + wm * 32 + (lane & 31);
   const int col = blockIdx.x * 64 + wn * 32 + (lane & 31);
   const int kh = 8 * (lane >> 5);
-  const bool rok = row < g.M, cok = col < g.N;
-  const __bf16* ap = g.A + (int64_t)(rok ? row : 0) * g.lda + kh;
-  const __bf16* bp = g.B + (int64_t)(cok ? col : 0) * g.ldb + kh;
+  const __bf16* ap = g.A + (int64_t)(row < g.M ? row : 0) * g.lda + kh;
+  const __bf16* bp = g.B + (int64_t)(col < g.N ? col : 0) * g.ldb + kh;
   f32x16 acc = {};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    const bf16x8 a = rok ? *(const bf16x8*)(ap + k0) : bf16x8{};
-    const bf16x8 b = cok ? *(const bf16x8*)(bp + k0) : bf16x8{};
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  if constexpr (KS > 0) {
+    constexpr int kChunk = KS < kGemmChunk ? KS : kGemmChunk;
+#pragma unroll
+    for (int c = 0; c < KS; c += kChunk) {
+      bf16x8 a[kChunk], b[kChunk];
+#pragma unroll
+      for (int s = 0; s < kChunk; ++s) {
+        a[s] = *(const bf16x8*)(ap + 16 * (c + s));
+        b[s] = *(const bf16x8*)(bp + 16 * (c + s));
+      }
+#pragma unroll
+      for (int s = 0; s < kChunk; ++s)
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[s], b[s], acc, 0, 0, 0);
+    }
+  } else {
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+      const bf16x8 a = *(const bf16x8*)(ap + k0);
+      const bf16x8 b = *(const bf16x8*)(bp + k0);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
   }
-  epilogue(g, acc, blockIdx.y * 64 + wm * 32, col, lane, g.C);
+  EpiIn in;
+  epi_load(g, blockIdx.y * 64 + wm * 32, col, lane, in);
+  epi_store(g, acc, col, in);
 }
 
 // wt[n*K + k] = bf16(w[k*N + n]) for one K x N weight matrix.
@@ -764,7 +814,10 @@ static int32_t gemm(hipStream_t st, bool ta, bool tb, GemmArgs a, int splits = 1
       ((uintptr_t)a.A & 15) == 0 && ((uintptr_t)a.B & 15) == 0) {
     a.kc = a.K;
     dim3 grid((a.N + 63) / 64, (a.M + 63) / 64, 1);
-    hipLaunchKernelGGL(gemm_nt_kernel, grid, dim3(256), 0, st, a);
+    if (a.K == 16) hipLaunchKernelGGL(gemm_nt_kernel<1>, grid, dim3(256), 0, st, a);
+    else if (a.K == 128) hipLaunchKernelGGL(gemm_nt_kernel<8>, grid, dim3(256), 0, st, a);
+    else if (a.K == 256) hipLaunchKernelGGL(gemm_nt_kernel<16>, grid, dim3(256), 0, st, a);
+    else hipLaunchKernelGGL(gemm_nt_kernel<0>, grid, dim3(256), 0, st, a);
     return launch_status("sph_mlp/gemm_nt");
   }
   a.kc = (a.K + splits - 1) / splits;
