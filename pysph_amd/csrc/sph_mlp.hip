@@ -462,6 +462,117 @@ __global__ __launch_bounds__(256) void edge_fwd_kernel(EdgeArgs a) {
   a.A[(int64_t)i * a.lda + lane + 64] = (__bf16)s1;
 }
 
+#ifndef SPH_EDGE_MFMA
+#define SPH_EDGE_MFMA 1
+#endif
+
+// x = hi + lo with hi = bf16(x) and lo = bf16(x - hi): hi*w_hi + hi*w_lo +
+// lo*w_hi reproduces an FP32 product to ~2^-16 relative.
+__device__ __forceinline__ void split_bf16(float x, __bf16& hi, __bf16& lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// The aggregation A_i = sum_j ReLU(P_i + b + Q_j + e_ij W_c) on the matrix
+// cores.  A wave takes the edges of node i 32 at a time: the 32 x 16 block
+// of edge features times W_c (16 x 128) is four v_mfma_f32_32x32x16_bf16
+// tiles of 32 hidden units, each as three products of the features' and
+// W_c's BF16 high and low parts (the FP32 products to ~2^-16).  In the
+// product's layout, lane l holds hidden unit 32t + (l & 31) of the 16 edge
+// rows 8(r / 4) + 4(l / 32) + (r % 4); it adds P_i + b and Q_j of each row,
+// applies the ReLU and sums its rows, and the two half-waves' row sums are
+// added at the end.  Rows past the node's last edge read that edge and are
+// masked out of the sums.  The block splits W_c into its BF16 fragments in
+// LDS once (8 KB, in the lanes' fragment order, so each read is one
+// conflict-free ds_read_b128); then each wave takes nodes w, w + nwaves, ...
+#ifndef SPH_EDGE_WAVES
+#define SPH_EDGE_WAVES 3  // 154 VGPRs: three waves per SIMD without spills
+#endif
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SPH_EDGE_WAVES)))
+void edge_fwd_mfma_kernel(EdgeArgs a) {
+  __shared__ bf16x8 wfrag[2 * 4 * 64];  // [hi, lo][tile][lane]
+  for (int f = threadIdx.x; f < 2 * 4 * 64; f += 256) {
+    const int l = f & 63, t = (f >> 6) & 3, lo = f >> 8;
+    const int c = 32 * t + (l & 31), k0 = 8 * (l >> 5);
+    bf16x8 v;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      __bf16 h, l2;
+      split_bf16(a.Wc[(k0 + s) * kH + c], h, l2);
+      v[s] = lo ? l2 : h;
+    }
+    wfrag[f] = v;
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(blockIdx.x * 4 + (threadIdx.x >> 6));
+  const int nwaves = gridDim.x * 4;
+  const int m = lane & 31, kh = 8 * (lane >> 5);
+  const int mr0 = 4 * (lane >> 5);  // edge row of r is mr0 + 8(r / 4) + r % 4
+  float be[4];
+#pragma unroll
+  for (int t = 0; t < 4; ++t) be[t] = a.be[32 * t + m];
+  for (int i = wave; i < a.n; i += nwaves) {
+    const int64_t e0 = a.off[i], e1 = a.off[i + 1];
+    float pb[4], sum[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      pb[t] = a.PQ[(int64_t)i * 2 * kH + 32 * t + m] + be[t];
+      sum[t] = 0.f;
+    }
+    for (int64_t eb = e0; eb < e1; eb += 32) {
+      const int64_t em = eb + m < e1 ? eb + m : e1 - 1;  // feature row of this lane
+      const float4 x0 = *(const float4*)(a.E + em * kE + kh);
+      const float4 x1 = *(const float4*)(a.E + em * kE + kh + 4);
+      const int jm = (int)a.nbr[em];
+      const int lim = (int)(e1 - eb);  // rows below lim are edges of node i
+      const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+      bf16x8 ah, al;
+#pragma unroll
+      for (int s = 0; s < 8; ++s) {
+        __bf16 h, l;
+        split_bf16(xs[s], h, l);
+        ah[s] = h;
+        al[s] = l;
+      }
+      int qoff[16];  // Q_j row of edge row r, in floats
+      float okf[16];  // 1 for the edges of node i, 0 for the rows past them
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int mr = mr0 + 8 * (r >> 2) + (r & 3);
+        const int j = __shfl(jm, mr);
+        qoff[r] = j * 2 * kH + kH + m;
+        okf[r] = mr < lim ? 1.f : 0.f;
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        float q[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) q[r] = a.PQ[(int64_t)qoff[r] + 32 * t];
+        const bf16x8 bh = wfrag[t * 64 + lane], bl = wfrag[256 + t * 64 + lane];
+        f32x16 acc = {};
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ah, bh, acc, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          // a masked row's z is finite (it re-reads a valid edge): x 0 adds 0
+          const float z = acc[r] + pb[t] + q[r];
+          sum[t] = fmaf(fmaxf(z, 0.f), okf[r], sum[t]);
+        }
+        // one tile's 16 neighbour-row loads in flight at a time: hoisting all
+        // 64 spills at the occupancy set above
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const float tot = sum[t] + __shfl_xor(sum[t], 32);
+      if (lane < 32) a.A[(int64_t)i * a.lda + 32 * t + m] = (__bf16)tot;
+    }
+  }
+}
+
 // Persistent: each wave walks nodes i = wave, wave + nwaves, ...  and keeps
 // its dWc / dbe partial sums in registers.
 __global__ __launch_bounds__(256) void edge_bwd_kernel(EdgeArgs a) {
@@ -809,6 +920,13 @@ static int32_t ensure(Buf& b, size_t bytes) {
 
 static int nblk(int64_t n) { return (int)((n + 255) / 256); }
 
+// Persistent edge aggregation: up to SPH_EDGE_WAVES waves per SIMD of the
+// 256 CUs, each wave then walking nodes with stride gridDim.x * 4.
+static int edge_fwd_blocks(int64_t n) {
+  const int64_t waves = std::min<int64_t>(n, 256 * 4 * SPH_EDGE_WAVES);
+  return (int)std::max<int64_t>(1, (waves + 3) / 4);
+}
+
 static int32_t gemm(hipStream_t st, bool ta, bool tb, GemmArgs a, int splits = 1) {
   if (!ta && tb && splits == 1 && a.K % 16 == 0 && a.lda % 8 == 0 && a.ldb % 8 == 0 &&
       ((uintptr_t)a.A & 15) == 0 && ((uintptr_t)a.B & 15) == 0) {
@@ -1122,7 +1240,11 @@ int32_t sph_mlp_forward(void* model, void* stream, const sph_mlp_graph* gr, floa
     e.A = HA + kH;
     e.lda = 2 * kH;
     T.begin(2);
-    hipLaunchKernelGGL(edge_fwd_kernel, dim3((int)((n + 3) / 4)), dim3(256), 0, st, e);
+    // the MFMA kernel indexes Q rows with 32-bit offsets (n * 2 * kH < 2^31)
+    if (SPH_EDGE_MFMA && n < (int64_t(1) << 22))
+      hipLaunchKernelGGL(edge_fwd_mfma_kernel, dim3(edge_fwd_blocks(n)), dim3(256), 0, st, e);
+    else
+      hipLaunchKernelGGL(edge_fwd_kernel, dim3((int)((n + 3) / 4)), dim3(256), 0, st, e);
     T.end();
     SPH_TRY(launch_status("sph_mlp_forward/edges"));
     {  // U = ReLU([H | A] W_u + c_u); H' = H + U

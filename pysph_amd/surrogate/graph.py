@@ -64,10 +64,18 @@ class ParticleGraph:
         self.overflow = torch.zeros(1, dtype=torch.int32, device=self.device)
         self.struct = _abi.SphMlpGraph()
 
-    def build(self):
+    def build(self, edges=True):
+        """Bin the array, export the radius-2h CSR and sum the density.
+        With edges=False only the binning and the summation density run
+        (what a density check needs); the CSR and the network's graph
+        struct keep the last full build."""
         pa, cm = self.pa, self.cm
         cm.update()
         stream = require_gpu(self.off)
+        if not edges:
+            _ops.density(cm, self.prm, pa, [pa], pa.get_carray("rho"),
+                         override=_ops.packed_override([pa]))
+            return self
         rmax = 2.0 * cm.max_h
         v = pa.view()
         _abi.call("sph_neighbor_counts", stream, cm.grid, v, v, 2.0, self.counts.data_ptr())

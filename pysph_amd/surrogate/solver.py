@@ -135,7 +135,7 @@ class SurrogateWCSPH:
         """Summation density (user order) of a state (default: current)."""
         if st is not None:
             self.set_state(st)
-        self.graph.build()
+        self.graph.build(edges=False)
         rho = get_user(self.graph.pa, "rho")
         self.graph.check_overflow()
         return rho
