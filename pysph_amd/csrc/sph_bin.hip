@@ -527,6 +527,9 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
   int ring = !(rd < (double)full) ? full : (int)rd;
   if (ring < 1) ring = 1;
   int64_t k = FILL ? a.off[i] : 0;
+  // FILL: whether the ids arrive ascending (then the row needs no sort)
+  int64_t last_id = INT64_MIN;
+  bool ascending = true;
   const View<T>& S = a.src;
   for_each_image(g, (double)xd, (double)yd, (double)zd, cx, cy, cz, ring * g.cs,
                  [&](double ox, double oy, double oz, int icx, int icy, int icz, bool image) {
@@ -547,7 +550,12 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
                              (double)xd, (double)yd, (double)zd, r2);
         }
         if (hit) {
-          if (FILL && (a.overflow == nullptr || k < a.cap)) a.out[k] = a.ids[j];
+          if (FILL) {
+            const int64_t id = a.ids[j];
+            ascending = ascending && id > last_id;
+            last_id = id;
+            if (a.overflow == nullptr || k < a.cap) a.out[k] = id;
+          }
           ++k;
         }
       }
@@ -560,9 +568,10 @@ __global__ __launch_bounds__(kBlock) void nbr_kernel(Grid g, NbrArgs<T> a) {
       *a.overflow = 1;  // plain vector store of a flag
       k = a.cap;
     }
-    // sort the row ascending (rows hold tens of entries)
+    // sort the row ascending (rows hold tens of entries); a row whose ids
+    // arrived ascending (slot ids of cell-ordered sources) is already sorted
     const int64_t b = a.off[i];
-    for (int64_t p = b + 1; p < k; ++p) {
+    for (int64_t p = b + 1; !ascending && p < k; ++p) {
       const int64_t v = a.out[p];
       int64_t q = p - 1;
       while (q >= b && a.out[q] > v) {
