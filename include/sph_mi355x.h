@@ -569,7 +569,11 @@ typedef struct sph_mlp_graph {
     int32_t dim;          /* 2 or 3 */
     /* SPH_MLP_INFER: sph_mlp_forward for inference only; it does not store
      * the activations sph_mlp_backward reads, which then returns SPH_EINVAL
-     * until a forward pass without the flag */
+     * until a forward pass without the flag.  SPH_MLP_EDGE_FP32: aggregate
+     * the edges with FP32 FMAs on the vector units instead of split-BF16
+     * products on the matrix cores (the path graphs of 2^22 nodes or more
+     * always take; the products agree to ~2^-16, the outputs to ~1e-3, as
+     * the aggregate's BF16 copy can round either way) */
     int32_t flags;
     const float *pos4;    /* (x, y, z, h) per node */
     const float *vel4;    /* (u, v, w, m) per node */
@@ -580,6 +584,7 @@ typedef struct sph_mlp_graph {
     double rho0, co, gamma, u_scale, g, rho_scale, alpha, eta;
 } sph_mlp_graph;
 #define SPH_MLP_INFER 1
+#define SPH_MLP_EDGE_FP32 2
 int64_t sph_mlp_param_count(void);
 int32_t sph_mlp_create(void **model);
 int32_t sph_mlp_destroy(void *model);

@@ -105,6 +105,7 @@ SPH_MLP_NIN = 16
 SPH_MLP_NE = 16
 SPH_MLP_ROUNDS = 2
 SPH_MLP_INFER = 1
+SPH_MLP_EDGE_FP32 = 2
 
 
 class SphMlpGraph(ctypes.Structure):

@@ -1298,7 +1298,7 @@ int32_t sph_mlp_forward(void* model, void* stream, const sph_mlp_graph* gr, floa
     e.lda = 2 * kH;
     T.begin(2);
     // the MFMA kernel indexes Q rows with 32-bit offsets (n * 2 * kH < 2^31)
-    if (SPH_EDGE_MFMA && n < (int64_t(1) << 22))
+    if (SPH_EDGE_MFMA && !(gr->flags & SPH_MLP_EDGE_FP32) && n < (int64_t(1) << 22))
       hipLaunchKernelGGL(edge_fwd_mfma_kernel, dim3(edge_fwd_blocks(n)), dim3(256), 0, st, e);
     else
       hipLaunchKernelGGL(edge_fwd_kernel, dim3((int)((n + 3) / 4)), dim3(256), 0, st, e);

@@ -101,14 +101,16 @@ class EdgeMLP:
         return dict(zip(("features", "gemm", "edges", "decode"), list(ms))), calls.value
 
     # ---- compute
-    def forward(self, graph, out=None, infer=False):
+    def forward(self, graph, out=None, infer=False, edge_fp32=False):
         """Predicted accelerations / g, [n x 2] FP32 device tensor.  With
         `infer` the pass keeps none of the activations a backward pass would
-        read (loss_backward runs its own forward pass)."""
+        read (loss_backward runs its own forward pass); `edge_fp32` takes the
+        FP32 vector-unit edge aggregation instead of the matrix-core one."""
         n = graph.n
         if out is None:
             out = torch.empty((max(n, 1), 2), dtype=torch.float32, device=graph.device)
-        graph.struct.flags = _abi.SPH_MLP_INFER if infer else 0
+        graph.struct.flags = ((_abi.SPH_MLP_INFER if infer else 0) |
+                              (_abi.SPH_MLP_EDGE_FP32 if edge_fp32 else 0))
         _abi.call("sph_mlp_forward", self.handle, require_gpu(out), ctypes.byref(graph.struct),
                   out.data_ptr())
         return out

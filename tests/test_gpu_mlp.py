@@ -101,6 +101,28 @@ def test_forward_matches_oracle(hdx):
     assert np.abs(Y).max() > 1e-3
 
 
+def test_fp32_edge_aggregation_matches_matrix_core():
+    """The FP32 vector-unit edge aggregation (the path that graphs of 2^22
+    nodes or more take) against the matrix-core one on the same graph, and
+    against the BF16-rounded oracle."""
+    g, _ = _graph(hdx=1.8, max_per=96)
+    theta = _params(1)
+    model = EdgeMLP(params=theta)
+    y_mc = model.forward(g)[:g.n].double().cpu().numpy()
+    y_32 = model.forward(g, edge_fp32=True)[:g.n].double().cpu().numpy()
+    pos4, vel4, rho, bt, off, nbr, c = _host(g)
+    X, E, dst, src = M.features(pos4, vel4, rho, bt, off, nbr, c)
+    p = M.unflatten(theta.astype(np.float64))
+    Yb, _ = M.forward(p, X, E, dst, src, g.n, round_bf16=True)
+    d, errb = _rel(y_32, y_mc), _rel(y_32, Yb)
+    print("FP32 vs matrix-core aggregation: %.3e; FP32 path vs BF16-rounded oracle: %.3e"
+          % (d, errb))
+    # the split-BF16 products differ from FP32 FMAs by ~2^-16, and the BF16
+    # copy of the aggregate can round the other way
+    assert 0 < d < 1e-3
+    assert errb < 3e-3
+
+
 def test_inference_forward_same_output_and_no_backward():
     """A forward pass for inference (SPH_MLP_INFER: the activations of the
     backward pass are not stored) gives the same bits as a training forward
